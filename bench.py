@@ -1,0 +1,132 @@
+"""Benchmark of the acoustic RK4 hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1|c5]
+                    [--cpu-sample-steps S] [--no-cpu]
+
+A step is one RK4 call (timestep.f90:41-86): five fused stages plus the
+explicit filter over the whole six-panel grid, state resident in HBM.
+value = 6*n*n*nr*K / (wall seconds of K steps), grid-point RK4 steps/s.
+
+Printed as one JSON line, with
+  roofline     : the dominant kernel (the fused stage kernel), algorithmic
+                 bytes per launch / its mean launch time (HIP events on the
+                 library's stream), against the 8 TB/s HBM peak; plus the whole
+                 step against 800 B per point per step (step_frac)
+  cpu_baseline : the CPU restatement of the reference (oracle/) on this host,
+                 on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-point RK4 steps/s (FP64, % HBM roofline) at 1 GPU; fwd+adjoint vs plain fwd cost"
+HBM_PEAK = 8.0e12
+# Algorithmic HBM bytes per grid point (FP64 state of 5 fields = 40 B):
+#   stage 1 and 5: read q + write q and qw / read q and qw + write q -> 120 B
+#   stages 2-4: read q and qw, write q and qw -> 160 B;  filter: 80 B
+STAGE_BYTES = (120, 160, 160, 160, 120)
+STEP_BYTES = 800
+
+DESCR = {
+    "c1": "c1: Gaussian pulse, uniform background, 6x32x32x64, FP64",
+    "c2": "c2: stratified background, spherical-harmonic mode + noise, sponge, 6x128x128x256, FP64",
+    "c3": "c3: stratified background, spherical-harmonic mode + noise, sponge, 6x256x256x384, FP64",
+    "c5": "c5: stratified background, spherical-harmonic mode + noise, sponge, 6x96x96x192, FP64",
+}
+
+
+def _cpu_baseline(cfg, bg, q0, dt, sample_steps):
+    import numpy as np  # noqa: F401
+    cores = min(16, os.cpu_count() or 1)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cores = int(os.environ["OMP_NUM_THREADS"])
+    import oracle
+    o = oracle.Oracle(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    o.rk4(q0, 1, dt)  # first touch / warm-up
+    t = time.perf_counter()
+    o.rk4(q0, sample_steps, dt)
+    el = time.perf_counter() - t
+    return {"value": cfg.points * sample_steps / el, "unit": "grid-point RK4 steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{sample_steps} RK4 step(s) of the {cfg.name} workload ({cfg.points} points) "
+                      f"after one warm-up step, C restatement of the reference with OpenMP, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, nothing runs it)", file=sys.stderr)
+        sys.exit(2)
+
+    from solar_amd import CONFIGS, Solver, make_inputs
+    cfg = CONFIGS[args.workload]
+    steps = args.steps if args.steps is not None else cfg.steps
+    bg, q0, dt, info = make_inputs(cfg)
+
+    s = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    s.set_state(q0)
+    s.rk4(args.warmup, dt)
+    s.sync()
+    t0 = time.perf_counter()
+    s.rk4_async(steps, dt)
+    s.sync()
+    wall = time.perf_counter() - t0
+    ok = s.check_finite()
+    s.close()
+    rate = cfg.points * steps / wall
+
+    # Kernel times with HIP events bracketing every launch on the library's
+    # stream (a separate handle: SOLAR_PROFILE is read at init).
+    os.environ["SOLAR_PROFILE"] = "1"
+    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    os.environ.pop("SOLAR_PROFILE")
+    p.set_state(q0)
+    p.rk4(2, dt)
+    p.rk4(args.profile_steps, dt)
+    kt = p.kernel_times()
+    p.close()
+    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points
+    stage_s = kt["stage_ms"] * 1e-3
+    achieved = stage_bytes / stage_s
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_stage_{cfg.name}.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_stage (fused RK stage)", "kernel_ms": kt["stage_ms"],
+                "algorithmic_bytes_per_launch": stage_bytes,
+                "step_frac": rate * STEP_BYTES / HBM_PEAK,
+                "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(cfg, bg, q0, dt, args.cpu_sample_steps)
+
+    out = {"metric": METRIC, "value": rate, "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": DESCR[cfg.name], "n": cfg.n, "nr": cfg.nr, "panels": 6, "points": cfg.points,
+                      "dt": dt, "seed": cfg.seed, "state_finite": ok},
+           "roofline": roofline, "cpu_baseline": cpu}
+    if cpu:
+        out["gpu_over_cpu"] = rate / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,454 @@
+// C ABI of the acoustic RK4 hot path (declarations and reference citations in
+// include/solar_amd.h).  One handle owns the device tables, the state
+// q(n,n,nr,5,6) and its work buffers on one HIP device and one stream.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/solar_amd.h"
+#include "solar_kernels.hpp"
+#include "solar_tables.hpp"
+
+using solar::DevTables;
+using solar::LaunchShape;
+using solar::RadialRow;
+using solar::Route;
+using solar::Tables;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(SOLAR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct solar_ctx {
+  int n = 0, nr = 0, device = 0;
+  size_t nf = 0;  // points per field per panel
+  hipStream_t stream = nullptr;
+  Tables tab;
+  DevTables dt{};
+  std::vector<void*> allocs;
+  double* q[2] = {nullptr, nullptr};
+  double* qw = nullptr;
+  double *gxp = nullptr, *gyp = nullptr, *gxv = nullptr, *gyv = nullptr;
+  double *fgx = nullptr, *fgy = nullptr;
+  double* acc = nullptr;
+  LaunchShape shape;
+  // hipGraph of one RK4 step for a given dt
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  // event timing
+  bool profile = false;
+  std::vector<hipEvent_t> ev;
+  double ktime[5] = {0, 0, 0, 0, 0};
+  size_t bytes = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(solar_ctx* c, T** p, size_t count) {
+  void* v = nullptr;
+  const size_t b = count * sizeof(T);
+  hipError_t e = hipMalloc(&v, b);
+  if (e != hipSuccess)
+    return fail(SOLAR_ERR_HIP, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+  c->allocs.push_back(v);
+  c->bytes += b;
+  *p = static_cast<T*>(v);
+  return SOLAR_OK;
+}
+
+template <typename T>
+int upload(solar_ctx* c, const T** dst, const std::vector<T>& src) {
+  T* d = nullptr;
+  int rc = dev_alloc(c, &d, src.size());
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(d, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  *dst = d;
+  return SOLAR_OK;
+}
+
+void release(solar_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->allocs.clear();
+}
+
+// Enqueues one RK4 step (timestep.f90:61-82) on the handle's stream.  The
+// state lives in q[0] before and after: five stages ping-pong q[0] <-> q[1],
+// the filter brings it back.  With `timed`, events bracket every launch.
+int enqueue_step(solar_ctx* c, double dt, bool timed) {
+  const DevTables& T = c->dt;
+  double* cur = c->q[0];
+  double* nxt = c->q[1];
+  int e = 0;
+  for (int s = 0; s < 5; ++s) {
+    if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+    HIPCHK(solar::launch_gather(T, cur, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+    if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+    HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream, c->shape));
+    double* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+  HIPCHK(solar::launch_gather(T, cur, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
+  if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+  HIPCHK(solar::launch_filter(T, cur, nxt, c->fgx, c->fgy, c->stream, c->shape));
+  if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+  return SOLAR_OK;
+}
+
+int build_graph(solar_ctx* c, double dt) {
+  if (c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+  if (c->graph) {
+    (void)hipGraphDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(c, dt, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIPCHK(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+  c->graph_dt = dt;
+  return SOLAR_OK;
+}
+
+bool env_flag(const char* name) {
+  const char* v = std::getenv(name);
+  return v && v[0] && std::strcmp(v, "0") != 0;
+}
+
+int env_int(const char* name) {
+  const char* v = std::getenv(name);
+  return v ? std::atoi(v) : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* solar_last_error(void) { return g_err.c_str(); }
+
+int solar_init(solar_ctx** out, int n, int nr, const double* gr, const double* c2, const double* rho0,
+               const double* g0, int sponge_len, double a_sponge) {
+  if (!out || !gr || !c2 || !rho0 || !g0) return fail(SOLAR_ERR_ARG, "solar_init: null argument");
+  *out = nullptr;
+  solar_ctx* c = new solar_ctx();
+  std::string err;
+  if (!solar::build_tables(n, nr, gr, c2, rho0, g0, sponge_len, a_sponge, &c->tab, &err)) {
+    delete c;
+    return fail(SOLAR_ERR_ARG, "solar_init: " + err);
+  }
+  c->n = n;
+  c->nr = nr;
+  c->nf = static_cast<size_t>(n) * n * nr;
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev < 1) {
+    delete c;
+    return fail(SOLAR_ERR_HIP, "solar_init: no HIP device available");
+  }
+  (void)hipGetDevice(&c->device);
+  int rc = SOLAR_OK;
+  auto bail = [&](int code) {
+    release(c);
+    delete c;
+    return code;
+  };
+  if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipStreamCreate failed"));
+
+  const Tables& t = c->tab;
+  DevTables& d = c->dt;
+  d.n = n;
+  d.nr = nr;
+  if ((rc = upload(c, &d.ci, t.metric_c)) || (rc = upload(c, &d.dj, t.metric_d)) || (rc = upload(c, &d.dc, t.dc)) ||
+      (rc = upload(c, &d.tdx, t.tdx)) || (rc = upload(c, &d.tde, t.tde)) || (rc = upload(c, &d.xy, t.xy)) ||
+      (rc = upload(c, &d.rad, t.radial)) || (rc = upload(c, &d.conv, t.conv)) || (rc = upload(c, &d.gw, t.gather_w)) ||
+      (rc = upload(c, &d.trans, t.trans)))
+    return bail(rc);
+  std::vector<Route> routes(24);
+  for (int p = 0; p < 6; ++p)
+    for (int e = 0; e < 4; ++e) routes[p * 4 + e] = t.route[p][e];
+  if ((rc = upload(c, &d.route, routes))) return bail(rc);
+  for (int q = 0; q < 7; ++q) {
+    d.fcx[q] = t.fcx[q];
+    d.fcy[q] = t.fcy[q];
+    d.filt[q] = solar::kFiltCoeffs[q];
+  }
+  for (int q = 0; q < 11; ++q) d.filtr[q] = solar::kFiltCoeffsR[q];
+
+  const size_t nq = c->nf * 30;
+  const size_t ng = static_cast<size_t>(12) * nr * n * 3;  // one field's ghosts, 6 panels x 2 sides
+  if ((rc = dev_alloc(c, &c->q[0], nq)) || (rc = dev_alloc(c, &c->q[1], nq)) || (rc = dev_alloc(c, &c->qw, nq)) ||
+      (rc = dev_alloc(c, &c->gxp, ng)) || (rc = dev_alloc(c, &c->gyp, ng)) || (rc = dev_alloc(c, &c->gxv, ng)) ||
+      (rc = dev_alloc(c, &c->gyv, ng)) || (rc = dev_alloc(c, &c->fgx, 5 * ng)) ||
+      (rc = dev_alloc(c, &c->fgy, 5 * ng)) || (rc = dev_alloc(c, &c->acc, 2)))
+    return bail(rc);
+  if (hipMemset(c->q[0], 0, nq * sizeof(double)) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipMemset failed"));
+  c->use_graph = !env_flag("SOLAR_NO_GRAPH");
+  c->profile = env_flag("SOLAR_PROFILE");
+  c->shape.stage_chunks = env_int("SOLAR_STAGE_CHUNKS");
+  c->shape.filter_chunks = env_int("SOLAR_FILTER_CHUNKS");
+  if (c->profile) {
+    c->ev.resize(13);
+    for (auto& e : c->ev)
+      if (hipEventCreate(&e) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipEventCreate failed"));
+  }
+  *out = c;
+  g_err.clear();
+  return SOLAR_OK;
+}
+
+int solar_finalize(solar_ctx* c) {
+  if (!c) return SOLAR_OK;
+  (void)hipStreamSynchronize(c->stream);
+  release(c);
+  delete c;
+  return SOLAR_OK;
+}
+
+int solar_set_state(solar_ctx* c, const double* q) {
+  if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_set_state: null argument");
+  HIPCHK(hipMemcpyAsync(c->q[0], q, c->nf * 30 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_get_state(solar_ctx* c, double* q) {
+  if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_get_state: null argument");
+  HIPCHK(hipMemcpyAsync(q, c->q[0], c->nf * 30 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_set_state_device(solar_ctx* c, const double* q) {
+  if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_set_state_device: null argument");
+  HIPCHK(hipMemcpyAsync(c->q[0], q, c->nf * 30 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_get_state_device(solar_ctx* c, double* q) {
+  if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_get_state_device: null argument");
+  HIPCHK(hipMemcpyAsync(q, c->q[0], c->nf * 30 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_rk4_async(solar_ctx* c, int nsteps, double dt) {
+  if (!c || nsteps < 0) return fail(SOLAR_ERR_ARG, "solar_rk4_async: invalid argument");
+  if (c->profile) {
+    for (double& v : c->ktime) v = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+      int rc = enqueue_step(c, dt, true);
+      if (rc) return rc;
+      HIPCHK(hipEventSynchronize(c->ev[12]));
+      float ms = 0.f;
+      for (int st = 0; st < 5; ++st) {
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * st], c->ev[2 * st + 1]));
+        c->ktime[1] += ms;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * st + 1], c->ev[2 * st + 2]));
+        c->ktime[0] += ms;
+      }
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[10], c->ev[11]));
+      c->ktime[3] += ms;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[11], c->ev[12]));
+      c->ktime[2] += ms;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[12]));
+      c->ktime[4] += ms;
+    }
+    if (nsteps > 0) {
+      c->ktime[0] /= 5.0 * nsteps;
+      c->ktime[1] /= 5.0 * nsteps;
+      c->ktime[2] /= nsteps;
+      c->ktime[3] /= nsteps;
+      c->ktime[4] /= nsteps;
+    }
+    return SOLAR_OK;
+  }
+  if (c->use_graph) {
+    if (!c->graph_exec || c->graph_dt != dt) {
+      int rc = build_graph(c, dt);
+      if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+    return SOLAR_OK;
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = enqueue_step(c, dt, false);
+    if (rc) return rc;
+  }
+  return SOLAR_OK;
+}
+
+int solar_sync(solar_ctx* c) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_sync: null handle");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_rk4(solar_ctx* c, int nsteps, double dt, double* time) {
+  int rc = solar_rk4_async(c, nsteps, dt);
+  if (rc) return rc;
+  rc = solar_sync(c);
+  if (rc) return rc;
+  if (time)
+    for (int s = 0; s < nsteps; ++s) *time = *time + dt;  // timestep.f90:82, once per step
+  return SOLAR_OK;
+}
+
+int solar_compute_rhs(solar_ctx* c, const double* q, double* q_rhs, int flags) {
+  if (!c || !q || !q_rhs) return fail(SOLAR_ERR_ARG, "solar_compute_rhs: null argument");
+  const size_t nq = c->nf * 30;
+  // q[1] and qw are free between steps; the state in q[0] is not touched.
+  HIPCHK(hipMemcpyAsync(c->q[1], q, nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+  const int mode = 4 | (flags & (SOLAR_RHS_BC | SOLAR_RHS_SPONGE));
+  HIPCHK(solar::launch_stage(c->dt, c->q[1], c->qw, c->qw, c->gxp, c->gyp, c->gxv, c->gyv, -1, 0.0, mode, c->stream,
+                             c->shape));
+  HIPCHK(hipMemcpyAsync(q_rhs, c->qw, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_filter_exp(solar_ctx* c, const double* q, double* qf) {
+  if (!c || !q || !qf) return fail(SOLAR_ERR_ARG, "solar_filter_exp: null argument");
+  const size_t nq = c->nf * 30;
+  HIPCHK(hipMemcpyAsync(c->q[1], q, nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
+  HIPCHK(solar::launch_filter(c->dt, c->q[1], c->qw, c->fgx, c->fgy, c->stream, c->shape));
+  HIPCHK(hipMemcpyAsync(qf, c->qw, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+static int ghosts_common(solar_ctx* c, const double* f0, const double* f1, double* gx, double* gy, bool vec) {
+  const size_t nf6 = c->nf * 6;
+  const size_t ng = static_cast<size_t>(12) * c->nr * c->n * 3;
+  // Stage the panel fields into q[1] at the field slots the gather reads,
+  // using qw as the upload buffer.
+  HIPCHK(hipMemcpyAsync(c->qw, f0, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, vec ? 1 : 4, c->stream));
+  if (vec) {
+    HIPCHK(hipMemcpyAsync(c->qw, f1, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 2, c->stream));
+    // the scalar slot (field 4) feeds the unused scalar output
+    HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 4, c->stream));
+  }
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+  HIPCHK(hipMemcpyAsync(gx, vec ? c->gxv : c->gxp, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(gy, vec ? c->gyv : c->gyp, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_scalar_ghosts(solar_ctx* c, const double* f, double* gx, double* gy) {
+  if (!c || !f || !gx || !gy) return fail(SOLAR_ERR_ARG, "solar_scalar_ghosts: null argument");
+  // vector slots must hold finite values: zero them via the placement of f
+  HIPCHK(hipMemcpyAsync(c->qw, f, c->nf * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 1, c->stream));
+  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 2, c->stream));
+  return ghosts_common(c, f, nullptr, gx, gy, false);
+}
+
+int solar_vector_ghosts(solar_ctx* c, const double* vxi, const double* veta, double* gx, double* gy) {
+  if (!c || !vxi || !veta || !gx || !gy) return fail(SOLAR_ERR_ARG, "solar_vector_ghosts: null argument");
+  return ghosts_common(c, vxi, veta, gx, gy, true);
+}
+
+int solar_check_finite(solar_ctx* c) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_check_finite: null handle");
+  HIPCHK(solar::launch_scan(c->q[0], c->nf, c->acc, 0, c->stream));
+  double h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (h[0] != 0.0)
+    return fail(SOLAR_ERR_NONFINITE, "solar_check_finite: " + std::to_string(static_cast<long long>(h[0])) +
+                                         " non-finite values in the state");
+  return SOLAR_OK;
+}
+
+int solar_norm_vr(solar_ctx* c, double* rms) {
+  if (!c || !rms) return fail(SOLAR_ERR_ARG, "solar_norm_vr: null argument");
+  HIPCHK(solar::launch_scan(c->q[0], c->nf, c->acc, 1, c->stream));
+  double h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *rms = std::sqrt(h[1] / static_cast<double>(c->nf * 6));
+  return SOLAR_OK;
+}
+
+int solar_kernel_times(solar_ctx* c, double* out) {
+  if (!c || !out) return fail(SOLAR_ERR_ARG, "solar_kernel_times: null argument");
+  if (!c->profile) return fail(SOLAR_ERR_ARG, "solar_kernel_times: set SOLAR_PROFILE=1 before solar_init");
+  for (int q = 0; q < 5; ++q) out[q] = c->ktime[q];
+  return SOLAR_OK;
+}
+
+int solar_device_bytes(solar_ctx* c, size_t* bytes) {
+  if (!c || !bytes) return fail(SOLAR_ERR_ARG, "solar_device_bytes: null argument");
+  *bytes = c->bytes;
+  return SOLAR_OK;
+}
+
+int solar_host_tables(int n, int nr, const double* gr, const double* c2, const double* rho0, const double* g0,
+                      int sponge_len, double a_sponge, double* scal, double* n1, double* r1, double* n2,
+                      double* interp, double* ghost, double* sigma, double* drho0dr) {
+  Tables t;
+  std::string err;
+  if (!solar::build_tables(n, nr, gr, c2, rho0, g0, sponge_len, a_sponge, &t, &err))
+    return fail(SOLAR_ERR_ARG, "solar_host_tables: " + err);
+  scal[0] = t.dxi;
+  scal[1] = t.deta;
+  scal[2] = t.dchi;
+  scal[3] = t.pi;
+  const std::vector<double>* a1[8] = {&t.gxi, &t.geta, &t.tanxi, &t.taneta, &t.metric_c, &t.metric_d, &t.global_c,
+                                      &t.global_d};
+  for (int a = 0; a < 8; ++a) std::memcpy(n1 + static_cast<size_t>(a) * n, a1[a]->data(), sizeof(double) * n);
+  std::memcpy(r1, t.gchi.data(), sizeof(double) * nr);
+  std::memcpy(r1 + nr, t.drdchi.data(), sizeof(double) * nr);
+  std::memcpy(r1 + 2 * nr, t.gr.data(), sizeof(double) * nr);
+  const std::vector<double>* a2[4] = {&t.mdelta, &t.sqrtd, &t.xy, &t.cd};
+  const size_t n2s = static_cast<size_t>(n) * n;
+  for (int a = 0; a < 4; ++a) std::memcpy(n2 + a * n2s, a2[a]->data(), sizeof(double) * n2s);
+  std::memcpy(interp, t.interpolant.data(), sizeof(double) * 27 * n);
+  const double* g[8] = {t.xm, t.xp, t.ym, t.yp, t.cm, t.cp, t.dm, t.dp};
+  for (int a = 0; a < 8; ++a)
+    for (int l = 1; l <= 3; ++l) ghost[a * 3 + l - 1] = g[a][l];
+  for (int k = 0; k < nr; ++k) {
+    sigma[k] = t.radial[k].sigma;
+    drho0dr[k] = t.radial[k].drho0dr;
+  }
+  return SOLAR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Interface between the host C-ABI layer and the device kernels.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include "solar_tables.hpp"
+
+namespace solar {
+
+// Device-resident views of the tables (all pointers are device pointers).
+struct DevTables {
+  int n, nr;
+  const double* ci;      // metric_C(i)            [n]
+  const double* dj;      // metric_D(j)            [n]
+  const double* dc;      // metric_delta/CD        [n*n]
+  const double* tdx;     // tan(xi)/metric_D       [n*n]
+  const double* tde;     // tan(eta)/metric_C      [n*n]
+  const double* xy;      // XY                     [n*n]
+  const RadialRow* rad;  // per-plane constants    [nr]
+  const double* conv;    // basis conversion       [3][n*n][6]
+  const double* gw;      // gather weights         [3][n][9]
+  const double* trans;   // receiver transform     [6][4][3][n][2]
+  const Route* route;    // panel routes           [6][4]
+  double fcx[7], fcy[7];
+  double filt[7], filtr[11];
+};
+
+// Launch-shape overrides (0 = automatic).
+struct LaunchShape {
+  int stage_chunks = 0;
+  int filter_chunks = 0;
+};
+
+hipError_t launch_gather(const DevTables& T, const double* q, bool vec, int sfield, int nfields, double* gxs,
+                         double* gys, double* gxv, double* gyv, hipStream_t st);
+hipError_t launch_stage(const DevTables& T, const double* qin, double* qout, double* qw, const double* gxp,
+                        const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
+                        hipStream_t st, const LaunchShape& ls);
+hipError_t launch_filter(const DevTables& T, const double* qin, double* qout, const double* fgx, const double* fgy,
+                         hipStream_t st, const LaunchShape& ls);
+hipError_t launch_scan(const double* q, size_t nf, double* acc, int mode, hipStream_t st);
+hipError_t launch_place(const double* src, double* dst, size_t nf, int dfield, hipStream_t st);
+
+}  // namespace solar

@@ -1,0 +1,241 @@
+"""ctypes binding of libsolar_amd.so (the C ABI of include/solar_amd.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsolar_amd.so")
+
+RHS_BC = 1
+RHS_SPONGE = 2
+# header:68-76
+ALPHA_RK4 = (0.0, -4.801594388477952e-01, -1.404247195200064e+00, -2.016477077503357e+00,
+             -1.056444269767207e+00)
+BETA_RK4 = (1.028639988104959e-01, 7.408540575766301e-01, 7.426530946683799e-01,
+            4.694937902357905e-01, 1.881733382887932e-01)
+
+_D = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+# (name, restype, argtypes) of every entry point declared in include/solar_amd.h
+EXPORTS = [
+    ("solar_init", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _D, _D, _D, _D,
+                                  ctypes.c_int, ctypes.c_double]),
+    ("solar_finalize", ctypes.c_int, [ctypes.c_void_p]),
+    ("solar_set_state", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_get_state", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_set_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("solar_get_state_device", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("solar_rk4", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]),
+    ("solar_rk4_async", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    ("solar_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("solar_compute_rhs", ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_int]),
+    ("solar_filter_exp", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("solar_scalar_ghosts", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D]),
+    ("solar_vector_ghosts", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D]),
+    ("solar_check_finite", ctypes.c_int, [ctypes.c_void_p]),
+    ("solar_norm_vr", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_kernel_times", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("solar_last_error", ctypes.c_char_p, []),
+    ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
+                                         _D, _D, _D, _D, _D, _D, _D, _D]),
+    ("solar_background", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
+    ("solar_panel_points", ctypes.c_int, [ctypes.c_int, _D]),
+    ("solar_generate_ic", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_uint64, _D, _D,
+                                         _D]),
+]
+
+
+class SolarError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SolarError(f"{path} not found: build it with `python -m solar_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, res, args in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f64(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load_library().solar_last_error().decode()
+        raise SolarError(f"{what} failed ({rc}): {msg}")
+
+
+def background(kind, nr):
+    """Radial grid and background profiles (gr, c2, rho0, g0); kind 0 uniform, 1 stratified."""
+    lib = load_library()
+    out = [np.zeros(nr) for _ in range(4)]
+    _check(lib.solar_background(kind, nr, *[_p(a) for a in out]), "solar_background")
+    return tuple(out)
+
+
+def panel_points(n):
+    """Unit position vectors of all grid points, shape (6, n, n, 3) indexed [p, j, i]."""
+    lib = load_library()
+    xyz = np.zeros((6, n, n, 3))
+    _check(lib.solar_panel_points(n, _p(xyz)), "solar_panel_points")
+    return xyz
+
+
+def generate_ic(kind, n, nr, gr, c2, seed, params=None):
+    """Seeded initial state q of shape (6, 5, nr, n, n) and a 4-vector of IC info."""
+    lib = load_library()
+    q = np.zeros((6, 5, nr, n, n))
+    info = np.zeros(4)
+    pp = None if params is None else _f64(params)
+    _check(lib.solar_generate_ic(kind, n, nr, _p(_f64(gr)), _p(_f64(c2)), ctypes.c_uint64(seed),
+                                 None if pp is None else _p(pp), _p(q), _p(info)), "solar_generate_ic")
+    return q, info
+
+
+def host_tables(n, nr, gr, c2, rho0, g0, sponge_len, a_sponge):
+    """Host tables exactly as the device uses them (no GPU needed)."""
+    lib = load_library()
+    out = dict(scal=np.zeros(4), n1=np.zeros(8 * n), r1=np.zeros(3 * nr), n2=np.zeros(4 * n * n),
+               interp=np.zeros(27 * n), ghost=np.zeros(24), sigma=np.zeros(nr), drho0dr=np.zeros(nr))
+    _check(lib.solar_host_tables(n, nr, _p(_f64(gr)), _p(_f64(c2)), _p(_f64(rho0)), _p(_f64(g0)), sponge_len,
+                                 a_sponge, *[_p(out[k]) for k in
+                                             ("scal", "n1", "r1", "n2", "interp", "ghost", "sigma", "drho0dr")]),
+           "solar_host_tables")
+    return out
+
+
+class Solver:
+    """The RK4 hot path on one GPU: state q(n,n,nr,5,6) resident in HBM.
+
+    Mirrors the reference's boundary: construct (the SETUP_* calls), set_state
+    (GENERATE_IC filling /RKWORK/ q), rk4 (call RK4, cube.f90:74), get_state.
+    Arrays are numpy float64 of shape (6, 5, nr, n, n) (= Fortran q(n,n,nr,5,6)).
+    """
+
+    def __init__(self, n, nr, gr, c2, rho0, g0, sponge_len, a_sponge=0.1):
+        self.lib = load_library()
+        self.n, self.nr = n, nr
+        self.shape = (6, 5, nr, n, n)
+        h = ctypes.c_void_p()
+        _check(self.lib.solar_init(ctypes.byref(h), n, nr, _p(_f64(gr)), _p(_f64(c2)), _p(_f64(rho0)),
+                                   _p(_f64(g0)), int(sponge_len), float(a_sponge)), "solar_init")
+        self.h = h
+        self.time = 0.0
+
+    def close(self):
+        if self.h:
+            self.lib.solar_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        a = _f64(a)
+        if a.size != int(np.prod(self.shape)):
+            raise SolarError(f"array of {a.size} elements, expected shape {self.shape}")
+        return a
+
+    def set_state(self, q):
+        _check(self.lib.solar_set_state(self.h, _p(self._arr(q))), "solar_set_state")
+
+    def get_state(self):
+        q = np.empty(self.shape)
+        _check(self.lib.solar_get_state(self.h, _p(q)), "solar_get_state")
+        return q
+
+    def set_state_device(self, ptr):
+        _check(self.lib.solar_set_state_device(self.h, ctypes.c_void_p(ptr)), "solar_set_state_device")
+
+    def get_state_device(self, ptr):
+        _check(self.lib.solar_get_state_device(self.h, ctypes.c_void_p(ptr)), "solar_get_state_device")
+
+    def rk4(self, nsteps, dt):
+        t = ctypes.c_double(self.time)
+        _check(self.lib.solar_rk4(self.h, int(nsteps), float(dt), ctypes.byref(t)), "solar_rk4")
+        self.time = t.value
+        return self.time
+
+    def rk4_async(self, nsteps, dt):
+        _check(self.lib.solar_rk4_async(self.h, int(nsteps), float(dt)), "solar_rk4_async")
+
+    def sync(self):
+        _check(self.lib.solar_sync(self.h), "solar_sync")
+
+    def compute_rhs(self, q, flags=0):
+        out = np.empty(self.shape)
+        _check(self.lib.solar_compute_rhs(self.h, _p(self._arr(q)), _p(out), int(flags)), "solar_compute_rhs")
+        return out
+
+    def filter_exp(self, q):
+        out = np.empty(self.shape)
+        _check(self.lib.solar_filter_exp(self.h, _p(self._arr(q)), _p(out)), "solar_filter_exp")
+        return out
+
+    def _gshapes(self):
+        n, nr = self.n, self.nr
+        return (6, 2, nr, n, 3), (6, 2, nr, 3, n)
+
+    def scalar_ghosts(self, f):
+        f = _f64(f)
+        sx, sy = self._gshapes()
+        gx, gy = np.empty(sx), np.empty(sy)
+        _check(self.lib.solar_scalar_ghosts(self.h, _p(f), _p(gx), _p(gy)), "solar_scalar_ghosts")
+        return gx, gy
+
+    def vector_ghosts(self, vxi, veta):
+        sx, sy = self._gshapes()
+        gx, gy = np.empty(sx), np.empty(sy)
+        _check(self.lib.solar_vector_ghosts(self.h, _p(_f64(vxi)), _p(_f64(veta)), _p(gx), _p(gy)),
+               "solar_vector_ghosts")
+        return gx, gy
+
+    def check_finite(self):
+        rc = self.lib.solar_check_finite(self.h)
+        if rc == -3:
+            return False
+        _check(rc, "solar_check_finite")
+        return True
+
+    def norm_vr(self):
+        v = ctypes.c_double()
+        _check(self.lib.solar_norm_vr(self.h, ctypes.byref(v)), "solar_norm_vr")
+        return v.value
+
+    def kernel_times(self):
+        out = np.zeros(5)
+        _check(self.lib.solar_kernel_times(self.h, _p(out)), "solar_kernel_times")
+        return dict(stage_ms=out[0], stage_gather_ms=out[1], filter_ms=out[2], filter_gather_ms=out[3],
+                    step_ms=out[4])
+
+    def device_bytes(self):
+        v = ctypes.c_size_t()
+        _check(self.lib.solar_device_bytes(self.h, ctypes.byref(v)), "solar_device_bytes")
+        return v.value

@@ -1,0 +1,311 @@
+"""Pins of the CPU restatement (oracle/) against what mathematics fixes.
+
+The reference ships no golden vectors, known-answer tests or fixtures for the
+solver, and its Fortran/MPI build is unbuildable here, so every check below is
+a closed form, an invariant or an independent geometric construction:
+
+  routing/interpolation  smooth fields sampled on the cubed sphere -> ghost
+                         values equal the field at the ghost points
+  vector transforms      tangent field -> receiver-basis components from the
+                         panel geometry (16 relations exact, the 8 reversed
+                         relations at the mirrored along-edge coordinate,
+                         which is what transform.f90:184-274 evaluates)
+  DDX/DDY/DDR            polynomial exactness and spectral accuracy
+  Lagrange weights       polynomial reproduction
+  filters                transfer function, constant preservation, the
+                         truncated radial sum 0.999998, zeroed end planes
+  gradient/divergence    closed-form fields
+  RK4 loop               4th-order amplification polynomial (Taylor series)
+  sponge/BC              single-precision sigma closed form, field selection
+"""
+import numpy as np
+import pytest
+
+import oracle
+from solar_amd import background, panel_points
+from tests.conftest import cube_points, grid, sph_basis, theta_phi_to_xieta
+
+N, R = 24, 12
+
+
+@pytest.fixture(scope="module")
+def orc():
+    gr, c2, rho0, g0 = background(0, R)
+    return oracle.Oracle(N, R, gr, c2, rho0, g0, 3)
+
+
+def _fun(x):
+    return x[..., 0] + 2 * x[..., 1] ** 2 + 3 * x[..., 2] + x[..., 0] * x[..., 2]
+
+
+def _field_on_grid(n, r, f):
+    g, _ = grid(n)
+    XI, ETA = np.meshgrid(g, g, indexing="xy")
+    out = np.zeros((6, r, n, n))
+    for p in range(6):
+        out[p] = f(cube_points(p, XI, ETA))[None]
+    return out
+
+
+def test_scalar_ghost_routing_matches_geometry(orc):
+    """All 24 directed panel relations, with reversal: ghost = field at the ghost point."""
+    g, d = grid(N)
+    f = _field_on_grid(N, R, _fun)
+    gx, gy = orc.scalar_ghosts(f)
+    lay = np.arange(1, 4)
+    for p in range(6):
+        for s, xl in enumerate([g[0] - lay * d, g[-1] + lay * d]):
+            XX, EE = np.meshgrid(xl, g, indexing="xy")
+            tru = _fun(cube_points(p, XX, EE))
+            # end points use the corner copy (interpolation.f90:304-305), not exact interpolation
+            assert np.abs(gx[p, s, 0] - tru)[1:-1].max() < 2e-7, (p, s)
+        for s, el in enumerate([g[0] - lay * d, g[-1] + lay * d]):
+            XX, EE = np.meshgrid(g, el, indexing="xy")
+            tru = _fun(cube_points(p, XX, EE))
+            assert np.abs(gy[p, s, 0] - tru)[:, 1:-1].max() < 2e-7, (p, s)
+
+
+def _tangent(x):
+    w = np.array([0.3, -0.5, 0.8])
+    c = np.array([0.2, 0.7, -0.4])
+    V = np.cross(w, x) + (c - (x @ c)[..., None] * x)
+    eth, eph, _ = sph_basis(x)
+    return (V * eth).sum(-1), (V * eph).sum(-1)
+
+
+# Receiver edges whose strip arrives reversed (derivative.f90:217-282,538-561).
+REVERSED = {(1, "S"), (2, "S"), (2, "N"), (3, "N"), (4, "W"), (4, "N"), (5, "E"), (5, "S")}
+
+
+def test_vector_ghost_transforms_match_geometry(orc):
+    g, d = grid(N)
+    XI, ETA = np.meshgrid(g, g, indexing="xy")
+    vxi = np.zeros((6, R, N, N))
+    veta = np.zeros((6, R, N, N))
+    for p in range(6):
+        vt, vp = _tangent(cube_points(p, XI, ETA))
+        a, b = theta_phi_to_xieta(p, XI, ETA, vt, vp)
+        vxi[p], veta[p] = a[None], b[None]
+    gx, gy = orc.vector_ghosts(vxi, veta)
+    lay = np.arange(1, 4)
+    n_rev = 0
+    for p in range(6):
+        for s, xl in enumerate([g[0] - lay * d, g[-1] + lay * d]):
+            XX, EE = np.meshgrid(xl, g, indexing="xy")
+            vt, vp = _tangent(cube_points(p, XX, EE))
+            rev = (p, "WE"[s]) in REVERSED
+            n_rev += rev
+            tru = theta_phi_to_xieta(p, XX, -EE if rev else EE, vt, vp)[0]
+            assert np.abs(gx[p, s, 0] - tru)[1:-1].max() < 5e-7, (p, s)
+            if rev:  # and the geometric receiver component differs at O(1)
+                geo = theta_phi_to_xieta(p, XX, EE, vt, vp)[0]
+                assert np.abs(gx[p, s, 0] - geo)[1:-1].max() > 1e-2
+        for s, el in enumerate([g[0] - lay * d, g[-1] + lay * d]):
+            XX, EE = np.meshgrid(g, el, indexing="xy")
+            vt, vp = _tangent(cube_points(p, XX, EE))
+            rev = (p, "SN"[s]) in REVERSED
+            n_rev += rev
+            tru = theta_phi_to_xieta(p, -XX if rev else XX, EE, vt, vp)[1]
+            assert np.abs(gy[p, s, 0] - tru)[:, 1:-1].max() < 5e-7, (p, s)
+    assert n_rev == 8
+
+
+def test_ddr_closed_forms():
+    nr = 40
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(16, nr, gr, c2, rho0, g0, 3)
+    # Tam-Dong closures and the DRP interior are exact on quadratics (to the
+    # 9-10 printed digits of header:40-55); r(chi) is linear.
+    d = o.ddr(np.stack([gr ** 2, gr ** 3], 1))
+    assert np.abs(d[:, 0] - 2 * gr).max() < 1e-6
+    assert np.abs(d[:, 1] - 3 * gr ** 2).max() < 1e-5
+    # smooth non-polynomial field, including both closures (testcode.f90:258-280)
+    s = o.ddr(np.sin(4 * gr)[:, None])[:, 0]
+    assert np.abs(s - 4 * np.cos(4 * gr)).max() < 2e-5
+
+
+def test_ddx_ddy_polynomial_and_ghost_indexing():
+    n, nr = 16, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    g, d = grid(n)
+    poly = lambda x: 0.3 + x - 2 * x ** 2 + 0.7 * x ** 3  # DRP interior is exact to cubic
+    dpoly = lambda x: 1 - 4 * x + 2.1 * x ** 2
+    lay = np.arange(1, 4)
+    f = np.broadcast_to(poly(g)[None, None, :], (nr, n, n)).copy()
+    fp = np.broadcast_to(poly(g[0] - lay * d)[None, None, :], (nr, n, 3)).copy()   # (3,N,R) [k][j][l]
+    fn = np.broadcast_to(poly(g[-1] + lay * d)[None, None, :], (nr, n, 3)).copy()
+    dx = o.ddx(f, fp, fn)
+    assert np.abs(dx - dpoly(g)[None, None, :]).max() < 1e-6
+    fy = np.transpose(f, (0, 2, 1)).copy()
+    gp = np.broadcast_to(poly(g[0] - lay * d)[None, :, None], (nr, 3, n)).copy()   # (N,3,R) [k][l][i]
+    gn = np.broadcast_to(poly(g[-1] + lay * d)[None, :, None], (nr, 3, n)).copy()
+    dy = o.ddy(fy, gp, gn)
+    assert np.abs(dy - dpoly(g)[None, :, None]).max() < 1e-6
+
+
+def test_lagrange_weights_reproduce_polynomials():
+    n, nr = 24, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    g, d = grid(n)
+    t = o.tables()
+    w = t["interp"].reshape(3, n, 9)
+    # weights of full 9-point stencils sum to 1; truncated ones too
+    assert np.abs(w[:, 1:-1].sum(-1) - 1).max() < 1e-12
+    lay = np.arange(1, 4)
+    target = np.arctan(np.tan(g)[None, :] / np.tan(g[-1] + lay[:, None] * d))  # metric.f90:89-91
+    for deg, sl in [(8, slice(4, n - 4)), (5, slice(1, n - 1))]:
+        poly = lambda x: sum((0.4 * x) ** q for q in range(deg + 1))
+        strip = np.broadcast_to(poly(g)[None, None, :], (nr, 3, n)).copy()  # (N,3,R)
+        out = o.interp_x(strip)
+        err = np.abs(out[0][:, sl] - poly(target)[:, sl]).max()
+        assert err < 1e-12, (deg, err)
+    # the corner copies (interpolation.f90:304-305)
+    strip = np.broadcast_to(g[None, None, :], (nr, 3, n)).copy()
+    out = o.interp_x(strip)
+    for l in range(3):
+        assert out[0, l, 0] == g[l + 1] and out[0, l, -1] == g[n - 2 - l]
+
+
+def test_filter_transfer_function():
+    n, nr = 32, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    idx = np.arange(n)
+    lay = np.arange(1, 4)
+    for kappa in [0.3, 1.1, 2.0, np.pi]:
+        f = np.broadcast_to(np.cos(kappa * idx)[None, None, :], (nr, n, n)).copy()
+        gp = np.broadcast_to(np.cos(kappa * (-lay))[None, None, :], (nr, n, 3)).copy()
+        gn = np.broadcast_to(np.cos(kappa * (n - 1 + lay))[None, None, :], (nr, n, 3)).copy()
+        out = o.filter_xi(f, gp, gn)
+        tf = 1 - np.sin(kappa / 2) ** 6  # [1,-6,15,44,15,-6,1]/64 (header:99-105)
+        assert np.abs(out - tf * f).max() < 1e-14
+
+
+def test_filter_exp_constant_field():
+    n, nr = 16, 24
+    gr, c2, rho0, g0 = background(1, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    q = np.full((6, 5, nr, n, n), 2.5)
+    qf = o.filter_exp(q)
+    # xi/eta filters preserve constants (dyadic coefficients sum to 1, ghost
+    # weights sum to 1); the radial filter's truncated decimals sum to
+    # 0.999998 (header:123-133); the 5 end planes are zeroed (filter.f90:652).
+    assert np.all(qf[:, :, :5] == 0) and np.all(qf[:, :, -5:] == 0)
+    assert np.abs(qf[:, :, 5:-5] - 2.5 * 0.999998).max() < 1e-13
+
+
+def test_rk4_loop_is_the_fourth_order_polynomial(t0_inputs):
+    """One oracle RK4 step equals F(P(dt L) q) with P(z) = sum_{m<=4} z^m/m! + O(z^5)."""
+    cfg, (gr, c2, rho0, g0), q0, dt, _ = t0_inputs
+    o = oracle.Oracle(cfg.n, cfg.nr, gr, c2, rho0, g0, cfg.sponge_len)
+    h = dt * 0.05
+    L = o.stage_rhs  # linear operator q -> rhs (COMPUTE_RHS, BC_RHS, SPONGE)
+    terms = [q0]
+    for m in range(1, 6):
+        terms.append(h * L(terms[-1]) / m)
+    q_taylor4 = sum(terms[:5])
+    step, _ = o.rk4(q0, 1, h)
+    ref = o.filter_exp(q_taylor4)
+    # residual is the z^5 term (gamma5 != 1/120) plus round-off
+    scale = np.linalg.norm(ref)
+    assert np.linalg.norm(step - ref) / scale < 5 * np.linalg.norm(terms[5]) / scale + 1e-13
+    # and a dropped 4th-order term would be visible
+    wrong = o.filter_exp(sum(terms[:4]))
+    assert np.linalg.norm(step - wrong) > 20 * np.linalg.norm(step - ref)
+
+
+def test_rk_coefficients_are_fourth_order():
+    a, b = oracle.rk_coeffs()
+    # amplification polynomial of the 2N scheme from its coefficients, for
+    # scalar z, compared with the Taylor series of exp(z)
+    zs = np.array([1e-2, 2e-2])
+    amp = []
+    for z in zs:
+        q, w = 1.0, 0.0
+        for s in range(5):
+            w = a[s] * w + z * q
+            q = q + b[s] * w
+        amp.append(q)
+    err = np.abs(np.array(amp) - np.exp(zs))
+    # O(z^5): halving z divides the error by ~32
+    assert err[1] / err[0] > 20 and err[1] < 1e-10
+
+
+def test_sponge_sigma_single_precision_and_fields(t0_inputs):
+    cfg, (gr, c2, rho0, g0), q0, dt, _ = t0_inputs
+    o = oracle.Oracle(cfg.n, cfg.nr, gr, c2, rho0, g0, cfg.sponge_len, 0.1)
+    L, nr = cfg.sponge_len, cfg.nr
+    for gk in range(1, nr + 1):
+        if gk >= nr - L + 1:
+            ratio = np.float32(gk - (nr - L + 1)) / np.float32(L)
+        elif gk <= L:
+            ratio = np.float32(L - gk) / np.float32(L)
+        else:
+            ratio = None
+        want = 0.0 if ratio is None else 0.1 * float(ratio) ** 3
+        assert o.sponge_sigma(gk) == pytest.approx(want, rel=1e-15, abs=0)
+    q = np.random.default_rng(0).standard_normal(q0.shape)
+    rhs = o.bc_rhs(o.compute_rhs(q))
+    full = o.stage_rhs(q)
+    sig = np.array([o.sponge_sigma(k + 1) for k in range(nr)])[None, :, None, None]
+    for f in range(5):
+        expect = rhs[:, f] - (sig * q[:, f] if f in (0, 3, 4) else 0.0)
+        assert np.abs(full[:, f] - expect).max() < 1e-12
+    # BC_RHS zeroes the first and last radial planes of every field
+    assert np.all(rhs[:, :, 0] == 0) and np.all(rhs[:, :, -1] == 0)
+
+
+def test_gradient_of_linear_field_closed_form():
+    n, nr = 32, 16
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    xyz = panel_points(n)  # (6, n, n, 3) [p, j, i]
+    a = np.array([0.4, -0.9, 0.6])
+    p = np.einsum("pjic,c->pji", xyz, a)[:, None] * gr[None, :, None, None]   # a . x
+    gx, ge, grr = o.gradient(p)
+    g, _ = grid(n)
+    XI, ETA = np.meshgrid(g, g, indexing="xy")
+    for pp in range(6):
+        eth, eph, er = sph_basis(cube_points(pp, XI, ETA))
+        wx, we = theta_phi_to_xieta(pp, XI, ETA, eth @ a, eph @ a)
+        assert np.abs(gx[pp] - wx[None]).max() < 2e-4, pp
+        assert np.abs(ge[pp] - we[None]).max() < 2e-4, pp
+        assert np.abs(grr[pp] - (er @ a)[None]).max() < 1e-8, pp
+
+
+def test_divergence_closed_forms():
+    n, nr = 32, 16
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    g, _ = grid(n)
+    XI, ETA = np.meshgrid(g, g, indexing="xy")
+    vxi = np.zeros((6, nr, n, n))
+    veta = np.zeros((6, nr, n, n))
+    cth = np.zeros((6, n, n))
+    for p in range(6):
+        x = cube_points(p, XI, ETA)
+        th = np.arccos(np.clip(x[..., 2], -1, 1))
+        a, b = theta_phi_to_xieta(p, XI, ETA, np.sin(th), 0 * th)  # v = sin(theta) e_theta
+        vxi[p], veta[p] = a[None], b[None]
+        cth[p] = np.cos(th)
+    div = o.divergence(vxi, veta, np.zeros_like(vxi))
+    want = 2 * cth[:, None] / gr[None, :, None, None]   # (1/(r sin)) d(sin^2)/dtheta
+    assert np.abs(div - want).max() < 1e-3
+    # radial field v = r e_r: divergence 3
+    div = o.divergence(np.zeros_like(vxi), np.zeros_like(vxi),
+                       np.broadcast_to(gr[None, :, None, None], vxi.shape).copy())
+    assert np.abs(div - 3).max() < 1e-6
+
+
+def test_rhs_linear_and_zero_preserving(t0_inputs):
+    cfg, (gr, c2, rho0, g0), q0, dt, _ = t0_inputs
+    o = oracle.Oracle(cfg.n, cfg.nr, gr, c2, rho0, g0, cfg.sponge_len)
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(q0.shape), rng.standard_normal(q0.shape)
+    lhs = o.stage_rhs(2.0 * a - 0.5 * b)
+    rhs = 2.0 * o.stage_rhs(a) - 0.5 * o.stage_rhs(b)
+    assert np.linalg.norm(lhs - rhs) / np.linalg.norm(rhs) < 1e-13
+    z, _ = o.rk4(np.zeros(q0.shape), 2, dt)
+    assert np.all(z == 0)
