@@ -39,72 +39,84 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
   }
 }
 
-// One thread per ghost value (receiver panel P, edge e, layer l, along index
-// m, plane k).  VEC: also the receiver-basis velocity ghost.
-//   q      : state, the scalar is field `sfield`, the vector fields 1, 2
-//   gx*, gy*: ghosts, x: [P][side][k][m][l], y: [P][side][k][l][m]
+// Ghost strips of one receiver edge line (receiver panel P, edge e, layer l,
+// plane k) per block.  The sender's source line (rows/columns 2..4 or
+// N-1..N-3 of grad.f90:28-31) is staged once in LDS -- for vectors already
+// converted to (theta, phi) with the sender's metric (transform.f90:19-56) --
+// then each thread interpolates one ghost value along the edge with the
+// 9-point Lagrange weights (interpolation.f90:140-378; unit weights encode the
+// end-point copies) and, for vectors, applies the receiver-basis transform
+// (transform.f90:161-274).  blockIdx.y selects a field offset (the filter's
+// five scalars).
+//   gx*: [P][side W/E][k][m][l],  gy*: [P][side S/N][k][l][m]
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_gather(DevTables T, const double* __restrict__ q, int sfield,
                                                 double* __restrict__ gxs, double* __restrict__ gys,
                                                 double* __restrict__ gxv, double* __restrict__ gyv,
                                                 long long field_stride_out) {
+  extern __shared__ double sline[];  // [3][n]: scalar, theta, phi
   const int n = T.n, nr = T.nr;
-  const long long total = 72LL * n * nr;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  const int fld = blockIdx.y;  // extra field offset (filter gather: 0..4)
-  const int m = (int)(t % n);
-  t /= n;
-  const int l = (int)(t % 3) + 1;
-  t /= 3;
-  const int e = (int)(t % 4);
-  t /= 4;
-  const int P = (int)(t % 6);
-  const int k = (int)(t / 6);
+  const int line = blockIdx.x;
+  const int k = line / 72;
+  const int rem = line % 72;
+  const int P = rem / 12;
+  const int e = (rem / 3) % 4;
+  const int l = rem % 3 + 1;
+  const int fld = blockIdx.y;
 
-  const int s = T.route[P * 4 + e].sender;
-  const int se = T.route[P * 4 + e].sender_edge;
-  const int mp = T.route[P * 4 + e].reverse ? n - 1 - m : m;
+  const Route rt = T.route[P * 4 + e];
+  const int s = rt.sender, se = rt.sender_edge;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
   const double* sp = q + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
-
-  double ps = 0.0, ths = 0.0, phs = 0.0;
   const double* vx = q + ((size_t)(s * 5 + 1) * nr + k) * plane;
   const double* ve = vx + nf;
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
-#pragma unroll
-  for (int tp = 0; tp < 9; ++tp) {
-    const int a = clampi(mp - 4 + tp, 0, n - 1);
+  double* ls = sline;
+  double* lt = sline + n;
+  double* lp = sline + 2 * n;
+  for (int a = threadIdx.x; a < n; a += blockDim.x) {
     const int idx = strip_index(se, l, a, n);
-    const double wt = w[tp];
-    ps = ps + sp[idx] * wt;
+    ls[a] = sp[idx];
     if (VEC) {
       const double* c = cv + (size_t)idx * 6;
       const double x = vx[idx], y = ve[idx];
-      const double th = c[0] * (c[1] * x + c[2] * y);
-      const double ph = c[3] * (c[4] * x + c[5] * y);
-      ths = ths + th * wt;
-      phs = phs + ph * wt;
+      lt[a] = c[0] * (c[1] * x + c[2] * y);
+      lp[a] = c[3] * (c[4] * x + c[5] * y);
     }
   }
-  size_t o;
+  __syncthreads();
   const size_t fo = (size_t)fld * field_stride_out;
-  if (e < 2) {
-    o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
-    gxs[o] = ps;
-  } else {
-    o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
-    gys[o] = ps;
-  }
-  if (VEC) {
-    const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
-    const double g = ab[0] * ths + ab[1] * phs;
-    if (e < 2)
-      gxv[o] = g;
-    else
-      gyv[o] = g;
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    const int mp = rt.reverse ? n - 1 - m : m;
+    const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
+    double ps = 0.0, ths = 0.0, phs = 0.0;
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) {
+      const int a = clampi(mp - 4 + tp, 0, n - 1);
+      const double wt = w[tp];
+      ps = ps + ls[a] * wt;
+      if (VEC) {
+        ths = ths + lt[a] * wt;
+        phs = phs + lp[a] * wt;
+      }
+    }
+    size_t o;
+    if (e < 2) {
+      o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
+      gxs[o] = ps;
+    } else {
+      o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
+      gys[o] = ps;
+    }
+    if (VEC) {
+      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
+      const double g = ab[0] * ths + ab[1] * phs;
+      if (e < 2)
+        gxv[o] = g;
+      else
+        gyv[o] = g;
+    }
   }
 }
 
@@ -116,27 +128,43 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return (b % 8) * (nb / 8) + b / 8;
 }
 
-__device__ __forceinline__ double pick7(const double (&w)[7], int c) {
-  // Select w[c] for a block-uniform c without dynamic register indexing.
-  double v = w[0];
-#pragma unroll
-  for (int q = 1; q < 7; ++q) v = (c == q) ? w[q] : v;
-  return v;
-}
+// Values one thread loads for one radial plane of the stage kernel: its own
+// point, the halo cells it fills, and the plane entering the radial window.
+struct StagePlane {
+  double vx, ve, q1, pc, rc;   // v_xi, v_eta, rho', p', v_r at (i,j,k)
+  double w[5];                 // qw at (i,j,k)
+  double hpx, hvx;             // i-halo cell (p', v_xi)
+  double hpy0, hve0, hpy1, hve1;  // low / high j-halo cells (p', v_eta)
+  double pn, rn;               // p', v_r of the plane entering the window
+};
 
-// One RK stage (or one RHS evaluation when rhs_mode is set).
+// One RK stage (or one RHS evaluation when rhs_mode is set), one pass over
+// HBM.  A block owns a TI x TJ column tile of one panel and marches through a
+// chunk of radial planes; p' and v_r sit in 7-plane register windows for the
+// radial stencils, the i/j neighbours of plane k come from a double-buffered
+// LDS tile, and plane k+1 is loaded into registers while plane k is computed.
 //   KIND 0: first stage (qw taken as 0, written), 1: middle stage (qw read and
 //   written), 2: last stage (qw read, not written back).
+// Occupancy floor of the stage kernel (waves per SIMD); 0 leaves it to the compiler.
+#ifndef SOLAR_STAGE_WAVES
+#define SOLAR_STAGE_WAVES 0
+#endif
+#if SOLAR_STAGE_WAVES > 0
+#define SOLAR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(SOLAR_STAGE_WAVES)))
+#else
+#define SOLAR_STAGE_ATTR
+#endif
+
 template <int TI, int TJ, int KIND>
-__global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __restrict__ qin,
+__global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, const double* __restrict__ qin,
                                                    double* __restrict__ qout, double* __restrict__ qw,
                                                    const double* __restrict__ gxp, const double* __restrict__ gyp,
                                                    const double* __restrict__ gxv, const double* __restrict__ gyv,
                                                    double alpha, double beta, double dt, int kchunk, int nkch,
                                                    int rhs_mode) {
-  __shared__ double sp[TJ + 6][TI + 6];
-  __shared__ double svx[TJ][TI + 6];
-  __shared__ double sve[TJ + 6][TI];
+  __shared__ double sp[2][TJ + 6][TI + 6];
+  __shared__ double svx[2][TJ][TI + 6];
+  __shared__ double sve[2][TJ + 6][TI];
 
   const int n = T.n, nr = T.nr;
   const int nti = n / TI, ntj = n / TJ;
@@ -165,15 +193,82 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
   double* o0 = qout + (size_t)(P * 5) * nf;
   double* w0 = qw + (size_t)(P * 5) * nf;
 
-  const double Ci = T.ci[i], Dj = T.dj[j];
-  const double dc = T.dc[ij], tdx = T.tdx[ij], tde = T.tde[ij], xy = T.xy[ij];
-
-  // Ghost rows of this panel.
+  // Halo cells this thread fills: source pointer and plane stride, from the
+  // field inside the panel or from the gathered ghost strips beyond its edge.
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
   const double* gxpW = gxp + (size_t)(P * 2) * gx_side;
   const double* gxvW = gxv + (size_t)(P * 2) * gx_side;
   const double* gypS = gyp + (size_t)(P * 2) * gy_side;
   const double* gyvS = gyv + (size_t)(P * 2) * gy_side;
+  const bool hx = tx < 3 || tx >= TI - 3, hy0 = ty < 3, hy1 = ty >= TJ - 3;
+  const double *hx_p = f4, *hx_v = f1, *hy0_p = f4, *hy0_v = f2, *hy1_p = f4, *hy1_v = f2;
+  size_t hx_s = plane, hy0_s = plane, hy1_s = plane;
+  int hx_col = 0;
+  if (tx < 3) {
+    const int ii = i0 - 3 + tx;
+    hx_col = tx;
+    if (ii >= 0) {
+      hx_p = f4 + (size_t)j * n + ii; hx_v = f1 + (size_t)j * n + ii;
+    } else {
+      hx_p = gxpW + (size_t)j * 3 + (-ii - 1); hx_v = gxvW + (size_t)j * 3 + (-ii - 1); hx_s = (size_t)n * 3;
+    }
+  } else if (tx >= TI - 3) {
+    const int ii = i + 3;
+    hx_col = tx + 6;
+    if (ii < n) {
+      hx_p = f4 + (size_t)j * n + ii; hx_v = f1 + (size_t)j * n + ii;
+    } else {
+      hx_p = gxpW + gx_side + (size_t)j * 3 + (ii - n); hx_v = gxvW + gx_side + (size_t)j * 3 + (ii - n);
+      hx_s = (size_t)n * 3;
+    }
+  }
+  if (hy0) {
+    const int jj = j0 - 3 + ty;
+    if (jj >= 0) {
+      hy0_p = f4 + (size_t)jj * n + i; hy0_v = f2 + (size_t)jj * n + i;
+    } else {
+      hy0_p = gypS + (size_t)(-jj - 1) * n + i; hy0_v = gyvS + (size_t)(-jj - 1) * n + i; hy0_s = (size_t)3 * n;
+    }
+  }
+  if (hy1) {
+    const int jj = j + 3;
+    if (jj < n) {
+      hy1_p = f4 + (size_t)jj * n + i; hy1_v = f2 + (size_t)jj * n + i;
+    } else {
+      hy1_p = gypS + gy_side + (size_t)(jj - n) * n + i; hy1_v = gyvS + gy_side + (size_t)(jj - n) * n + i;
+      hy1_s = (size_t)3 * n;
+    }
+  }
+
+  auto load_plane = [&](int k, StagePlane& L) {
+    const size_t kp = (size_t)k * plane + ij;
+    L.vx = f1[kp];
+    L.ve = f2[kp];
+    L.q1 = f0[kp];
+    L.pc = f4[kp];
+    L.rc = f3[kp];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L.w[f] = (KIND == 0) ? 0.0 : w0[f * nf + kp];
+    if (hx) {
+      L.hpx = hx_p[(size_t)k * hx_s];
+      L.hvx = hx_v[(size_t)k * hx_s];
+    }
+    if (hy0) {
+      L.hpy0 = hy0_p[(size_t)k * hy0_s];
+      L.hve0 = hy0_v[(size_t)k * hy0_s];
+    }
+    if (hy1) {
+      L.hpy1 = hy1_p[(size_t)k * hy1_s];
+      L.hve1 = hy1_v[(size_t)k * hy1_s];
+    }
+    // plane entering the radial window when the window slides before k
+    const int kn = min(k + 3, nr - 1);
+    L.pn = f4[(size_t)kn * plane + ij];
+    L.rn = f3[(size_t)kn * plane + ij];
+  };
+
+  const double Ci = T.ci[i], Dj = T.dj[j];
+  const double dc = T.dc[ij], tdx = T.tdx[ij], tde = T.tde[ij], xy = T.xy[ij];
 
   // Radial windows of p' and v_r over planes kb..kb+6.
   double pw[7], rw[7];
@@ -183,73 +278,38 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
     pw[q] = f4[(size_t)(kb + q) * plane + ij];
     rw[q] = f3[(size_t)(kb + q) * plane + ij];
   }
+  StagePlane nxt;
+  load_plane(k0, nxt);
 
   for (int k = k0; k < k1; ++k) {
+    const StagePlane cur = nxt;
+    if (k + 1 < k1) load_plane(k + 1, nxt);
     const int want = clampi(k - 3, 0, nr - 7);
-    if (want != kb) {  // slides by exactly one plane
+    if (want != kb) {  // slides by exactly one plane: plane k+3 enters
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         pw[q] = pw[q + 1];
         rw[q] = rw[q + 1];
       }
+      pw[6] = cur.pn;
+      rw[6] = cur.rn;
       kb = want;
-      pw[6] = f4[(size_t)(kb + 6) * plane + ij];
-      rw[6] = f3[(size_t)(kb + 6) * plane + ij];
     }
-    const int c = k - kb;
-    const double pc = pick7(pw, c);
-    const double rc = pick7(rw, c);
-    const size_t kp = (size_t)k * plane;
-    const double vx = f1[kp + ij];
-    const double ve = f2[kp + ij];
-
-    // Plane tile with halos; panel-edge halos come from the gathered ghosts.
-    sp[ty + 3][tx + 3] = pc;
-    svx[ty][tx + 3] = vx;
-    sve[ty + 3][tx] = ve;
-    if (tx < 3) {
-      const int ii = i0 - 3 + tx;
-      if (ii >= 0) {
-        sp[ty + 3][tx] = f4[kp + (size_t)j * n + ii];
-        svx[ty][tx] = f1[kp + (size_t)j * n + ii];
-      } else {
-        const size_t g = ((size_t)k * n + j) * 3 + (-ii - 1);
-        sp[ty + 3][tx] = gxpW[g];
-        svx[ty][tx] = gxvW[g];
-      }
+    const int bf = k & 1;
+    sp[bf][ty + 3][tx + 3] = cur.pc;
+    svx[bf][ty][tx + 3] = cur.vx;
+    sve[bf][ty + 3][tx] = cur.ve;
+    if (hx) {
+      sp[bf][ty + 3][hx_col] = cur.hpx;
+      svx[bf][ty][hx_col] = cur.hvx;
     }
-    if (tx >= TI - 3) {
-      const int ii = i + 3;
-      if (ii < n) {
-        sp[ty + 3][tx + 6] = f4[kp + (size_t)j * n + ii];
-        svx[ty][tx + 6] = f1[kp + (size_t)j * n + ii];
-      } else {
-        const size_t g = gx_side + ((size_t)k * n + j) * 3 + (ii - n);
-        sp[ty + 3][tx + 6] = gxpW[g];
-        svx[ty][tx + 6] = gxvW[g];
-      }
+    if (hy0) {
+      sp[bf][ty][tx + 3] = cur.hpy0;
+      sve[bf][ty][tx] = cur.hve0;
     }
-    if (ty < 3) {
-      const int jj = j0 - 3 + ty;
-      if (jj >= 0) {
-        sp[ty][tx + 3] = f4[kp + (size_t)jj * n + i];
-        sve[ty][tx] = f2[kp + (size_t)jj * n + i];
-      } else {
-        const size_t g = ((size_t)k * 3 + (-jj - 1)) * n + i;
-        sp[ty][tx + 3] = gypS[g];
-        sve[ty][tx] = gyvS[g];
-      }
-    }
-    if (ty >= TJ - 3) {
-      const int jj = j + 3;
-      if (jj < n) {
-        sp[ty + 6][tx + 3] = f4[kp + (size_t)jj * n + i];
-        sve[ty + 6][tx] = f2[kp + (size_t)jj * n + i];
-      } else {
-        const size_t g = gy_side + ((size_t)k * 3 + (jj - n)) * n + i;
-        sp[ty + 6][tx + 3] = gypS[g];
-        sve[ty + 6][tx] = gyvS[g];
-      }
+    if (hy1) {
+      sp[bf][ty + 6][tx + 3] = cur.hpy1;
+      sve[bf][ty + 6][tx] = cur.hve1;
     }
     __syncthreads();
 
@@ -259,13 +319,14 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
     const RadialRow& row = T.rad[k];
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-      dvx = dvx + T.fcx[q] * svx[ty][tx + q];
-      dve = dve + T.fcy[q] * sve[ty + q][tx];
-      dpx = dpx + T.fcx[q] * sp[ty + 3][tx + q];
-      dpe = dpe + T.fcy[q] * sp[ty + q][tx + 3];
+      dvx = dvx + T.fcx[q] * svx[bf][ty][tx + q];
+      dve = dve + T.fcy[q] * sve[bf][ty + q][tx];
+      dpx = dpx + T.fcx[q] * sp[bf][ty + 3][tx + q];
+      dpe = dpe + T.fcy[q] * sp[bf][ty + q][tx + 3];
       dvr = dvr + row.w[q] * rw[q];
       dpr = dpr + row.w[q] * pw[q];
     }
+    const double vx = cur.vx, ve = cur.ve, rc = cur.rc, pc = cur.pc, q1 = cur.q1;
 
     // grad.f90:125-133 and :49-57.
     const double div =
@@ -274,7 +335,6 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
     const double get = row.inv_r * (Ci * dpe + (xy * dpx) / Ci);
 
     // compute_rhs_acoustic.f90:38-149 (forcing omitted).
-    const double q1 = f0[kp + ij];
     double r1 = -(row.rho0 * div);
     r1 = r1 - rc * row.drho0dr;
     double r2 = -(gxi / row.rho0);
@@ -296,7 +356,7 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
       r5 = r5 - sg * (pc - 0.0);
     }
 
-    const size_t x = kp + ij;
+    const size_t x = (size_t)k * plane + ij;
     if (rhs_mode) {
       o0[x] = r1;
       o0[nf + x] = r2;
@@ -309,23 +369,28 @@ __global__ __launch_bounds__(TI* TJ) void k_stage(DevTables T, const double* __r
       const double qq[5] = {q1, vx, ve, rc, pc};
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
-        const double wold = (KIND == 0) ? 0.0 : w0[f * nf + x];
-        const double wn = alpha * wold + dt * rr[f];
+        const double wn = alpha * cur.w[f] + dt * rr[f];
         o0[f * nf + x] = qq[f] + beta * wn;
         if (KIND != 2) w0[f * nf + x] = wn;
       }
     }
-    __syncthreads();
   }
 }
 
 // FILTER_EXP for one field of one panel over a chunk of radial planes.
 //   fgx/fgy: ghosts of the unfiltered field (filter.f90:25-28), layouts as in
-//   k_gather, one block of 2*6 sides per field.
+//   k_gather, 12 sides (6 panels x 2) per field.
+// Per plane: the (TJ+6) x (TI+6) tile of the unfiltered field goes to LDS
+// (prefetched into registers one plane ahead), FILTER_XI runs on the in-panel
+// rows, FILTER_ETA on the result, and FILTER_R from an 11-plane register
+// window.
 template <int TI, int TJ>
 __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __restrict__ qin,
                                                     double* __restrict__ qout, const double* __restrict__ fgx,
                                                     const double* __restrict__ fgy, int kchunk, int nkch) {
+  constexpr int NT = TI * TJ;
+  constexpr int NCELL = (TJ + 6) * (TI + 6);
+  constexpr int NC = (NCELL + NT - 1) / NT;
   __shared__ double sq[TJ + 6][TI + 6];
   __shared__ double sx[TJ + 6][TI];
 
@@ -351,7 +416,7 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
   const double* g = qin + (size_t)PF * nf;
   double* o = qout + (size_t)PF * nf;
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
-  const size_t fstride = (size_t)12 * gx_side;  // one field's ghosts (6 panels x 2 sides)
+  const size_t fstride = (size_t)12 * gx_side;
   const double* gxW = fgx + F * fstride + (size_t)(P * 2) * gx_side;
   const double* gyS = fgy + F * fstride + (size_t)(P * 2) * gy_side;
   const size_t ij = (size_t)j * n + i;
@@ -362,31 +427,54 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
   const int a = max(k0, kNstr), bnd = min(k1, nr - kNstr);
   if (a >= bnd) return;
 
-  constexpr int NT = TI * TJ;
+  // Tile cells this thread fills, with their source and plane stride.  Corner
+  // cells beyond the panel in both directions are never read and stay 0.
   const int tid = ty * TI + tx;
+  const double* cptr[NC];
+  size_t cstr[NC];
+  bool cval[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int e = tid + c * NT;
+    cptr[c] = g;
+    cstr[c] = 0;
+    cval[c] = false;
+    if (e < NCELL) {
+      const int rr = e / (TI + 6), cc = e % (TI + 6);
+      const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
+      const bool jin = gj >= 0 && gj < n, iin = gi >= 0 && gi < n;
+      if (jin && iin) {
+        cptr[c] = g + (size_t)gj * n + gi; cstr[c] = plane; cval[c] = true;
+      } else if (jin) {
+        cptr[c] = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
+        cstr[c] = (size_t)n * 3; cval[c] = true;
+      } else if (iin) {
+        cptr[c] = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
+        cstr[c] = (size_t)3 * n; cval[c] = true;
+      }
+    }
+  }
+  double nx[NC];
+  auto load_plane = [&](int kk) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) nx[c] = cval[c] ? cptr[c][(size_t)kk * cstr[c]] : 0.0;
+  };
+
   double win[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) win[q] = 0.0;
 
-  for (int kk = a - kNstr; kk < bnd + kNstr; ++kk) {
-    const size_t kp = (size_t)kk * plane;
-    // Plane tile with halos (corners only where they lie inside the panel
-    // in j, since out-of-panel rows are not xi-filtered).
-    for (int e = tid; e < (TJ + 6) * (TI + 6); e += NT) {
-      const int rr = e / (TI + 6), cc = e % (TI + 6);
-      const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
-      double v = 0.0;
-      const bool jin = gj >= 0 && gj < n, iin = gi >= 0 && gi < n;
-      if (jin && iin) {
-        v = g[kp + (size_t)gj * n + gi];
-      } else if (jin) {
-        v = (gi < 0) ? gxW[((size_t)kk * n + gj) * 3 + (-gi - 1)]
-                     : gxW[gx_side + ((size_t)kk * n + gj) * 3 + (gi - n)];
-      } else if (iin) {
-        v = (gj < 0) ? gyS[((size_t)kk * 3 + (-gj - 1)) * n + gi]
-                     : gyS[gy_side + ((size_t)kk * 3 + (gj - n)) * n + gi];
-      }
-      sq[rr][cc] = v;
+  const int kbeg = a - kNstr, kend = bnd + kNstr;
+  load_plane(kbeg);
+  for (int kk = kbeg; kk < kend; ++kk) {
+    double cur[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cur[c] = nx[c];
+    if (kk + 1 < kend) load_plane(kk + 1);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = tid + c * NT;
+      if (e < NCELL) (&sq[0][0])[e] = cur[c];
     }
     __syncthreads();
     // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
@@ -418,7 +506,6 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
       for (int q = 0; q < 11; ++q) s = s + T.filtr[q] * win[q];
       o[(size_t)kout * plane + ij] = s;
     }
-    __syncthreads();
   }
 }
 
@@ -470,13 +557,14 @@ int chunks_for(int tiles, int nr, int target_blocks, int min_chunk) {
 
 hipError_t launch_gather(const DevTables& T, const double* q, bool vec, int sfield, int nfields, double* gxs,
                          double* gys, double* gxv, double* gyv, hipStream_t st) {
-  const long long total = 72LL * T.n * T.nr;
   const long long field_stride = 12LL * T.nr * T.n * 3;
-  dim3 grid((unsigned)((total + 255) / 256), (unsigned)nfields);
+  dim3 grid((unsigned)(72 * T.nr), (unsigned)nfields);
+  const int threads = T.n < 256 ? T.n : 256;
+  const size_t lds = (size_t)3 * T.n * sizeof(double);
   if (vec)
-    hipLaunchKernelGGL(k_gather<true>, grid, dim3(256), 0, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+    hipLaunchKernelGGL(k_gather<true>, grid, dim3(threads), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
   else
-    hipLaunchKernelGGL(k_gather<false>, grid, dim3(256), 0, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+    hipLaunchKernelGGL(k_gather<false>, grid, dim3(threads), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
   return hipGetLastError();
 }
 

@@ -5,7 +5,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsolar_amd.so")
+LIB_PATH = os.environ.get("SOLAR_AMD_LIB") or os.path.join(_HERE, "libsolar_amd.so")
 
 RHS_BC = 1
 RHS_SPONGE = 2

@@ -168,3 +168,27 @@ def test_graph_and_eager_launches_agree(t0, monkeypatch):
         s.set_state(q0)
         s.rk4(4, dt)
         assert np.array_equal(s.get_state(), a)
+
+
+@pytest.mark.parametrize("n,nr,kind,steps", [(48, 20, 1, 2), (64, 32, 1, 2), (128, 24, 0, 1)])
+def test_rk4_tile_shapes(n, nr, kind, steps):
+    """Every tile shape the launcher picks (16x16, 32x8, 64x4) and chunked radial marching."""
+    from solar_amd import background, generate_ic
+    bg = background(kind, nr)
+    L = max(1, round(0.08 * nr))
+    q0, _ = generate_ic(kind, n, nr, bg[0], bg[1], 99, [1e-3, 0.05 if kind == 0 else 1e-6, 20])
+    q0 = q0 + 1e-4 * np.random.default_rng(n).standard_normal(q0.shape)
+    dt = 0.5 * min(bg[0][1] - bg[0][0], bg[0][0] * (np.pi / 2) / (n - 1))
+    print(f"[case] tile n={n} nr={nr} kind={kind} steps={steps}", flush=True)
+    with Solver(n, nr, *bg, L, 0.1) as s:
+        s.set_state(q0)
+        s.rk4(steps, dt)
+        got = s.get_state()
+        o = oracle.Oracle(n, nr, *bg, L, 0.1)
+        want, _ = o.rk4(q0, steps, dt)
+        assert _assert_parity(got, want, f"rk4 n={n} nr={nr}")
+        q = np.random.default_rng(7).standard_normal(q0.shape)
+        assert _assert_parity(s.filter_exp(q), o.filter_exp(q), f"filter n={n}")
+        gx, gy = s.vector_ghosts(q[:, 1], q[:, 2])
+        ox, oy = o.vector_ghosts(q[:, 1], q[:, 2])
+        assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
