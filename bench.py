@@ -66,6 +66,11 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--state-bits", type=int, default=64, choices=(64, 32),
+                    help="storage of the state and 2N accumulator (arithmetic is FP64 either way)")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="also run STEPS steps in FP64 and FP32 storage and report the relative L2 error "
+                         "of each field of the FP32 run against the FP64 run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -77,7 +82,8 @@ def main():
     steps = args.steps if args.steps is not None else cfg.steps
     bg, q0, dt, info = make_inputs(cfg)
 
-    s = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    bits = args.state_bits
+    s = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits)
     s.set_state(q0)
     s.rk4(args.warmup, dt)
     s.sync()
@@ -92,14 +98,15 @@ def main():
     # Kernel times with HIP events bracketing every launch on the library's
     # stream (a separate handle: SOLAR_PROFILE is read at init).
     os.environ["SOLAR_PROFILE"] = "1"
-    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits)
     os.environ.pop("SOLAR_PROFILE")
     p.set_state(q0)
     p.rk4(2, dt)
     p.rk4(args.profile_steps, dt)
     kt = p.kernel_times()
     p.close()
-    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points
+    scale = bits / 64.0  # FP32 storage halves the algorithmic bytes
+    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points * scale
     stage_s = kt["stage_ms"] * 1e-3
     achieved = stage_bytes / stage_s
     traffic = None
@@ -110,8 +117,23 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "kernel": "k_stage (fused RK stage)", "kernel_ms": kt["stage_ms"],
                 "algorithmic_bytes_per_launch": stage_bytes,
-                "step_frac": rate * STEP_BYTES / HBM_PEAK,
+                "step_frac": rate * STEP_BYTES * scale / HBM_PEAK,
                 "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
+
+    precision = None
+    if args.precision_study:
+        import numpy as np
+        fin = {}
+        for b in (64, 32):
+            with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=b) as x:
+                x.set_state(q0)
+                x.rk4(args.precision_study, dt)
+                fin[b] = x.get_state()
+        errs = [float(np.linalg.norm(fin[32][:, f] - fin[64][:, f]) / np.linalg.norm(fin[64][:, f]))
+                for f in range(5)]
+        precision = {"steps": args.precision_study, "rel_l2_fp32_vs_fp64": dict(zip(
+            ["rho", "v_xi", "v_eta", "v_r", "p"], errs))}
+        del fin
 
     cpu = None
     if not args.no_cpu:
@@ -119,10 +141,12 @@ def main():
 
     out = {"metric": METRIC, "value": rate, "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
            "warmup": args.warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "vs_baseline": None, "dtype": "f64" if bits == 64 else "f32-state/f64-arith", "data": "synthetic",
            "config": {"workload": DESCR[cfg.name], "n": cfg.n, "nr": cfg.nr, "panels": 6, "points": cfg.points,
                       "dt": dt, "seed": cfg.seed, "state_finite": ok},
            "roofline": roofline, "cpu_baseline": cpu}
+    if precision:
+        out["precision"] = precision
     if cpu:
         out["gpu_over_cpu"] = rate / cpu["value"]
     print(json.dumps(out), flush=True)

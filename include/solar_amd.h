@@ -56,6 +56,16 @@ enum {
 int solar_init(solar_ctx** ctx, int n, int nr, const double* gr, const double* c2, const double* rho0,
                const double* g0, int sponge_len, double a_sponge);
 
+/* As solar_init, choosing the storage type of the state and of the 2N
+ * accumulator: state_bits = 64 (FP64, the reference's real*8) or 32 (FP32
+ * storage; all arithmetic, metric terms, interpolation weights and ghost
+ * strips stay FP64).  No reference counterpart: the reference is FP64 only. */
+int solar_init_ex(solar_ctx** ctx, int n, int nr, const double* gr, const double* c2, const double* rho0,
+                  const double* g0, int sponge_len, double a_sponge, int state_bits);
+
+/* Storage width of the handle's state (64 or 32). */
+int solar_state_bits(solar_ctx* ctx, int* bits);
+
 /* Releases the handle and its device memory (MPI_FINALIZE, cube.f90:94). */
 int solar_finalize(solar_ctx* ctx);
 
@@ -66,9 +76,11 @@ int solar_set_state(solar_ctx* ctx, const double* q);
 /* Copies the device state back to the host. */
 int solar_get_state(solar_ctx* ctx, double* q);
 
-/* Device-pointer state transfer (device-to-device copies on the handle's stream). */
-int solar_set_state_device(solar_ctx* ctx, const double* q_dev);
-int solar_get_state_device(solar_ctx* ctx, double* q_dev);
+/* Device-pointer state transfer (device-to-device copies on the handle's
+ * stream); the device array holds q(n,n,nr,5,6) in the handle's storage type
+ * (double, or float for a 32-bit handle). */
+int solar_set_state_device(solar_ctx* ctx, const void* q_dev);
+int solar_get_state_device(solar_ctx* ctx, void* q_dev);
 
 /* nsteps calls of RK4 (timestep.f90:41-86): five stages of
  * COMPUTE_RHS -> BC_RHS -> SPONGE -> 2N update, then FILTER_EXP, then

@@ -40,12 +40,14 @@ int fail(int code, const std::string& msg) {
 struct solar_ctx {
   int n = 0, nr = 0, device = 0;
   size_t nf = 0;  // points per field per panel
+  bool f32 = false;  // state stored as float (FP32-state variant)
+  size_t esz = 8;    // bytes per state element
   hipStream_t stream = nullptr;
   Tables tab;
   DevTables dt{};
   std::vector<void*> allocs;
-  double* q[2] = {nullptr, nullptr};
-  double* qw = nullptr;
+  void* q[2] = {nullptr, nullptr};  // state ping-pong, element type per f32
+  void* qw = nullptr;               // 2N accumulator, same type
   double *gxp = nullptr, *gyp = nullptr, *gxv = nullptr, *gyv = nullptr;
   double *fgx = nullptr, *fgy = nullptr;
   double* acc = nullptr;
@@ -101,22 +103,23 @@ void release(solar_ctx* c) {
 // the filter brings it back.  With `timed`, events bracket every launch.
 int enqueue_step(solar_ctx* c, double dt, bool timed) {
   const DevTables& T = c->dt;
-  double* cur = c->q[0];
-  double* nxt = c->q[1];
+  void* cur = c->q[0];
+  void* nxt = c->q[1];
   int e = 0;
   for (int s = 0; s < 5; ++s) {
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-    HIPCHK(solar::launch_gather(T, cur, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+    HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-    HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream, c->shape));
-    double* t = cur;
+    HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
+                               c->shape));
+    void* t = cur;
     cur = nxt;
     nxt = t;
   }
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-  HIPCHK(solar::launch_gather(T, cur, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
+  HIPCHK(solar::launch_gather(T, cur, c->f32, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-  HIPCHK(solar::launch_filter(T, cur, nxt, c->fgx, c->fgy, c->stream, c->shape));
+  HIPCHK(solar::launch_filter(T, cur, nxt, c->f32, c->fgx, c->fgy, c->stream, c->shape));
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
   return SOLAR_OK;
 }
@@ -160,7 +163,13 @@ const char* solar_last_error(void) { return g_err.c_str(); }
 
 int solar_init(solar_ctx** out, int n, int nr, const double* gr, const double* c2, const double* rho0,
                const double* g0, int sponge_len, double a_sponge) {
+  return solar_init_ex(out, n, nr, gr, c2, rho0, g0, sponge_len, a_sponge, 64);
+}
+
+int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double* c2, const double* rho0,
+                  const double* g0, int sponge_len, double a_sponge, int state_bits) {
   if (!out || !gr || !c2 || !rho0 || !g0) return fail(SOLAR_ERR_ARG, "solar_init: null argument");
+  if (state_bits != 64 && state_bits != 32) return fail(SOLAR_ERR_ARG, "solar_init: state_bits must be 64 or 32");
   *out = nullptr;
   solar_ctx* c = new solar_ctx();
   std::string err;
@@ -171,6 +180,8 @@ int solar_init(solar_ctx** out, int n, int nr, const double* gr, const double* c
   c->n = n;
   c->nr = nr;
   c->nf = static_cast<size_t>(n) * n * nr;
+  c->f32 = state_bits == 32;
+  c->esz = c->f32 ? sizeof(float) : sizeof(double);
   int ndev = 0;
   hipError_t he = hipGetDeviceCount(&ndev);
   if (he != hipSuccess || ndev < 1) {
@@ -208,16 +219,24 @@ int solar_init(solar_ctx** out, int n, int nr, const double* gr, const double* c
 
   const size_t nq = c->nf * 30;
   const size_t ng = static_cast<size_t>(12) * nr * n * 3;  // one field's ghosts, 6 panels x 2 sides
-  if ((rc = dev_alloc(c, &c->q[0], nq)) || (rc = dev_alloc(c, &c->q[1], nq)) || (rc = dev_alloc(c, &c->qw, nq)) ||
-      (rc = dev_alloc(c, &c->gxp, ng)) || (rc = dev_alloc(c, &c->gyp, ng)) || (rc = dev_alloc(c, &c->gxv, ng)) ||
+  char *q0 = nullptr, *q1 = nullptr, *qwb = nullptr;
+  if ((rc = dev_alloc(c, &q0, nq * c->esz)) || (rc = dev_alloc(c, &q1, nq * c->esz)) ||
+      (rc = dev_alloc(c, &qwb, nq * c->esz)) || (rc = dev_alloc(c, &c->gxp, ng)) || (rc = dev_alloc(c, &c->gyp, ng)) || (rc = dev_alloc(c, &c->gxv, ng)) ||
       (rc = dev_alloc(c, &c->gyv, ng)) || (rc = dev_alloc(c, &c->fgx, 5 * ng)) ||
       (rc = dev_alloc(c, &c->fgy, 5 * ng)) || (rc = dev_alloc(c, &c->acc, 2)))
     return bail(rc);
-  if (hipMemset(c->q[0], 0, nq * sizeof(double)) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipMemset failed"));
+  c->q[0] = q0;
+  c->q[1] = q1;
+  c->qw = qwb;
+  if (hipMemset(c->q[0], 0, nq * c->esz) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipMemset failed"));
   c->use_graph = !env_flag("SOLAR_NO_GRAPH");
   c->profile = env_flag("SOLAR_PROFILE");
   c->shape.stage_chunks = env_int("SOLAR_STAGE_CHUNKS");
   c->shape.filter_chunks = env_int("SOLAR_FILTER_CHUNKS");
+  c->shape.stage_ti = env_int("SOLAR_STAGE_TI");
+  c->shape.stage_tj = env_int("SOLAR_STAGE_TJ");
+  c->shape.filter_ti = env_int("SOLAR_FILTER_TI");
+  c->shape.filter_tj = env_int("SOLAR_FILTER_TJ");
   if (c->profile) {
     c->ev.resize(13);
     for (auto& e : c->ev)
@@ -236,31 +255,62 @@ int solar_finalize(solar_ctx* c) {
   return SOLAR_OK;
 }
 
+// Host double array -> device buffer in the handle's storage type.
+static int upload_state(solar_ctx* c, void* dst, const double* src, size_t count) {
+  if (!c->f32) {
+    HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::vector<float> tmp(src, src + count);
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), count * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SOLAR_OK;
+}
+
+// Device buffer in the handle's storage type -> host double array.
+static int download_state(solar_ctx* c, double* dst, const void* src, size_t count) {
+  if (!c->f32) {
+    HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(count);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t x = 0; x < count; ++x) dst[x] = tmp[x];
+  }
+  return SOLAR_OK;
+}
+
 int solar_set_state(solar_ctx* c, const double* q) {
   if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_set_state: null argument");
-  HIPCHK(hipMemcpyAsync(c->q[0], q, c->nf * 30 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc = upload_state(c, c->q[0], q, c->nf * 30);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return SOLAR_OK;
 }
 
 int solar_get_state(solar_ctx* c, double* q) {
   if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_get_state: null argument");
-  HIPCHK(hipMemcpyAsync(q, c->q[0], c->nf * 30 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return SOLAR_OK;
+  return download_state(c, q, c->q[0], c->nf * 30);
 }
 
-int solar_set_state_device(solar_ctx* c, const double* q) {
+int solar_set_state_device(solar_ctx* c, const void* q) {
   if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_set_state_device: null argument");
-  HIPCHK(hipMemcpyAsync(c->q[0], q, c->nf * 30 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->q[0], q, c->nf * 30 * c->esz, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return SOLAR_OK;
 }
 
-int solar_get_state_device(solar_ctx* c, double* q) {
+int solar_get_state_device(solar_ctx* c, void* q) {
   if (!c || !q) return fail(SOLAR_ERR_ARG, "solar_get_state_device: null argument");
-  HIPCHK(hipMemcpyAsync(q, c->q[0], c->nf * 30 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(q, c->q[0], c->nf * 30 * c->esz, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_state_bits(solar_ctx* c, int* bits) {
+  if (!c || !bits) return fail(SOLAR_ERR_ARG, "solar_state_bits: null argument");
+  *bits = c->f32 ? 32 : 64;
   return SOLAR_OK;
 }
 
@@ -330,41 +380,40 @@ int solar_compute_rhs(solar_ctx* c, const double* q, double* q_rhs, int flags) {
   if (!c || !q || !q_rhs) return fail(SOLAR_ERR_ARG, "solar_compute_rhs: null argument");
   const size_t nq = c->nf * 30;
   // q[1] and qw are free between steps; the state in q[0] is not touched.
-  HIPCHK(hipMemcpyAsync(c->q[1], q, nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(solar::launch_gather(c->dt, c->q[1], true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+  int rc = upload_state(c, c->q[1], q, nq);
+  if (rc) return rc;
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
   const int mode = 4 | (flags & (SOLAR_RHS_BC | SOLAR_RHS_SPONGE));
-  HIPCHK(solar::launch_stage(c->dt, c->q[1], c->qw, c->qw, c->gxp, c->gyp, c->gxv, c->gyv, -1, 0.0, mode, c->stream,
-                             c->shape));
-  HIPCHK(hipMemcpyAsync(q_rhs, c->qw, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return SOLAR_OK;
+  HIPCHK(solar::launch_stage(c->dt, c->q[1], c->qw, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, -1, 0.0, mode,
+                             c->stream, c->shape));
+  return download_state(c, q_rhs, c->qw, nq);
 }
 
 int solar_filter_exp(solar_ctx* c, const double* q, double* qf) {
   if (!c || !q || !qf) return fail(SOLAR_ERR_ARG, "solar_filter_exp: null argument");
   const size_t nq = c->nf * 30;
-  HIPCHK(hipMemcpyAsync(c->q[1], q, nq * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(solar::launch_gather(c->dt, c->q[1], false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
-  HIPCHK(solar::launch_filter(c->dt, c->q[1], c->qw, c->fgx, c->fgy, c->stream, c->shape));
-  HIPCHK(hipMemcpyAsync(qf, c->qw, nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return SOLAR_OK;
+  int rc = upload_state(c, c->q[1], q, nq);
+  if (rc) return rc;
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], c->f32, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
+  HIPCHK(solar::launch_filter(c->dt, c->q[1], c->qw, c->f32, c->fgx, c->fgy, c->stream, c->shape));
+  return download_state(c, qf, c->qw, nq);
 }
 
 static int ghosts_common(solar_ctx* c, const double* f0, const double* f1, double* gx, double* gy, bool vec) {
   const size_t nf6 = c->nf * 6;
   const size_t ng = static_cast<size_t>(12) * c->nr * c->n * 3;
   // Stage the panel fields into q[1] at the field slots the gather reads,
-  // using qw as the upload buffer.
-  HIPCHK(hipMemcpyAsync(c->qw, f0, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, vec ? 1 : 4, c->stream));
+  // using qw (at least 6*nf doubles in either precision) as the upload buffer.
+  double* up = static_cast<double*>(c->qw);
+  HIPCHK(hipMemcpyAsync(up, f0, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_place(up, c->q[1], c->f32, c->nf, vec ? 1 : 4, c->stream));
   if (vec) {
-    HIPCHK(hipMemcpyAsync(c->qw, f1, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 2, c->stream));
+    HIPCHK(hipMemcpyAsync(up, f1, nf6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(solar::launch_place(up, c->q[1], c->f32, c->nf, 2, c->stream));
     // the scalar slot (field 4) feeds the unused scalar output
-    HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 4, c->stream));
+    HIPCHK(solar::launch_place(up, c->q[1], c->f32, c->nf, 4, c->stream));
   }
-  HIPCHK(solar::launch_gather(c->dt, c->q[1], true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+  HIPCHK(solar::launch_gather(c->dt, c->q[1], c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
   HIPCHK(hipMemcpyAsync(gx, vec ? c->gxv : c->gxp, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(gy, vec ? c->gyv : c->gyp, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -373,10 +422,11 @@ static int ghosts_common(solar_ctx* c, const double* f0, const double* f1, doubl
 
 int solar_scalar_ghosts(solar_ctx* c, const double* f, double* gx, double* gy) {
   if (!c || !f || !gx || !gy) return fail(SOLAR_ERR_ARG, "solar_scalar_ghosts: null argument");
-  // vector slots must hold finite values: zero them via the placement of f
-  HIPCHK(hipMemcpyAsync(c->qw, f, c->nf * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 1, c->stream));
-  HIPCHK(solar::launch_place(c->qw, c->q[1], c->nf, 2, c->stream));
+  // vector slots must hold finite values: fill them with f as well
+  double* up = static_cast<double*>(c->qw);
+  HIPCHK(hipMemcpyAsync(up, f, c->nf * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(solar::launch_place(up, c->q[1], c->f32, c->nf, 1, c->stream));
+  HIPCHK(solar::launch_place(up, c->q[1], c->f32, c->nf, 2, c->stream));
   return ghosts_common(c, f, nullptr, gx, gy, false);
 }
 
@@ -387,7 +437,7 @@ int solar_vector_ghosts(solar_ctx* c, const double* vxi, const double* veta, dou
 
 int solar_check_finite(solar_ctx* c) {
   if (!c) return fail(SOLAR_ERR_ARG, "solar_check_finite: null handle");
-  HIPCHK(solar::launch_scan(c->q[0], c->nf, c->acc, 0, c->stream));
+  HIPCHK(solar::launch_scan(c->q[0], c->f32, c->nf, c->acc, 0, c->stream));
   double h[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -399,7 +449,7 @@ int solar_check_finite(solar_ctx* c) {
 
 int solar_norm_vr(solar_ctx* c, double* rms) {
   if (!c || !rms) return fail(SOLAR_ERR_ARG, "solar_norm_vr: null argument");
-  HIPCHK(solar::launch_scan(c->q[0], c->nf, c->acc, 1, c->stream));
+  HIPCHK(solar::launch_scan(c->q[0], c->f32, c->nf, c->acc, 1, c->stream));
   double h[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

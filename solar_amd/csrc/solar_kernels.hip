@@ -1,7 +1,8 @@
-// Device kernels of the acoustic RK4 hot path for gfx950 (MI355X), FP64.
+// Device kernels of the acoustic RK4 hot path for gfx950 (MI355X).
 //
 // State layout in HBM: q(n,n,nr,5,6), i fastest, one n*n*nr block per field
-// per panel.  All arithmetic follows the reference's expression order and is
+// per panel, stored as S = double (FP64 path) or float (FP32-state variant:
+// storage only -- every kernel computes in FP64 with FP64 tables and ghosts).  All arithmetic follows the reference's expression order and is
 // compiled with -ffp-contract=off, so results match the CPU restatement of
 // the reference bit for bit (tables are built on the host, see
 // solar_tables.cpp).
@@ -39,55 +40,71 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
   }
 }
 
-// Ghost strips of one receiver edge line (receiver panel P, edge e, layer l,
-// plane k) per block.  The sender's source line (rows/columns 2..4 or
-// N-1..N-3 of grad.f90:28-31) is staged once in LDS -- for vectors already
-// converted to (theta, phi) with the sender's metric (transform.f90:19-56) --
-// then each thread interpolates one ghost value along the edge with the
-// 9-point Lagrange weights (interpolation.f90:140-378; unit weights encode the
-// end-point copies) and, for vectors, applies the receiver-basis transform
-// (transform.f90:161-274).  blockIdx.y selects a field offset (the filter's
-// five scalars).
+// A plane-strided source of one value per radial plane: the state field
+// inside the panel (storage type S) or a gathered FP64 ghost strip beyond its
+// edge.
+template <typename S>
+struct Src {
+  const S* f = nullptr;
+  const double* g = nullptr;
+  size_t stride = 0;
+  __device__ __forceinline__ double at(int k) const {
+    return g ? g[(size_t)k * stride] : static_cast<double>(f[(size_t)k * stride]);
+  }
+};
+
+// Ghost strips of one receiver edge (receiver panel P, edge e, plane k) per
+// block, all three layers.  The sender's three source lines (rows/columns
+// 2..4 or N-1..N-3 of grad.f90:28-31) are staged once in LDS -- for vectors
+// already converted to (theta, phi) with the sender's metric
+// (transform.f90:19-56); threads walk the layer index fastest so the three
+// adjacent columns of a W/E strip share cache lines.  Each thread then
+// interpolates ghost values along the edge with the 9-point Lagrange weights
+// (interpolation.f90:140-378; unit weights encode the end-point copies) and,
+// for vectors, applies the receiver-basis transform (transform.f90:161-274).
+// blockIdx.y selects a field offset (the filter's five scalars).
 //   gx*: [P][side W/E][k][m][l],  gy*: [P][side S/N][k][l][m]
-template <bool VEC>
-__global__ __launch_bounds__(256) void k_gather(DevTables T, const double* __restrict__ q, int sfield,
+template <bool VEC, typename S>
+__global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
                                                 double* __restrict__ gxs, double* __restrict__ gys,
                                                 double* __restrict__ gxv, double* __restrict__ gyv,
                                                 long long field_stride_out) {
-  extern __shared__ double sline[];  // [3][n]: scalar, theta, phi
+  extern __shared__ double sline[];  // [3 layers][3 quantities][n]: scalar, theta, phi
   const int n = T.n, nr = T.nr;
-  const int line = blockIdx.x;
-  const int k = line / 72;
-  const int rem = line % 72;
-  const int P = rem / 12;
-  const int e = (rem / 3) % 4;
-  const int l = rem % 3 + 1;
+  const int edge = blockIdx.x;
+  const int k = edge / 24;
+  const int P = (edge % 24) / 4;
+  const int e = edge % 4;
   const int fld = blockIdx.y;
 
   const Route rt = T.route[P * 4 + e];
   const int s = rt.sender, se = rt.sender_edge;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const double* sp = q + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
-  const double* vx = q + ((size_t)(s * 5 + 1) * nr + k) * plane;
-  const double* ve = vx + nf;
+  const S* sp = q + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
+  const S* vx = q + ((size_t)(s * 5 + 1) * nr + k) * plane;
+  const S* ve = vx + nf;
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
-  double* ls = sline;
-  double* lt = sline + n;
-  double* lp = sline + 2 * n;
-  for (int a = threadIdx.x; a < n; a += blockDim.x) {
+  const int row = (se == kW || se == kE);  // column strips: layer index is the fast memory index
+  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+    const int l = row ? t % 3 + 1 : t / n + 1;
+    const int a = row ? t / 3 : t % n;
     const int idx = strip_index(se, l, a, n);
-    ls[a] = sp[idx];
+    double* ln = sline + (size_t)(l - 1) * 3 * n;
+    ln[a] = static_cast<double>(sp[idx]);
     if (VEC) {
       const double* c = cv + (size_t)idx * 6;
-      const double x = vx[idx], y = ve[idx];
-      lt[a] = c[0] * (c[1] * x + c[2] * y);
-      lp[a] = c[3] * (c[4] * x + c[5] * y);
+      const double x = static_cast<double>(vx[idx]), y = static_cast<double>(ve[idx]);
+      ln[n + a] = c[0] * (c[1] * x + c[2] * y);
+      ln[2 * n + a] = c[3] * (c[4] * x + c[5] * y);
     }
   }
   __syncthreads();
   const size_t fo = (size_t)fld * field_stride_out;
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+    const int l = (e < 2) ? t % 3 + 1 : t / n + 1;  // output layout: l fastest for x ghosts
+    const int m = (e < 2) ? t / 3 : t % n;
+    const double* ln = sline + (size_t)(l - 1) * 3 * n;
     const int mp = rt.reverse ? n - 1 - m : m;
     const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
     double ps = 0.0, ths = 0.0, phs = 0.0;
@@ -95,10 +112,10 @@ __global__ __launch_bounds__(256) void k_gather(DevTables T, const double* __res
     for (int tp = 0; tp < 9; ++tp) {
       const int a = clampi(mp - 4 + tp, 0, n - 1);
       const double wt = w[tp];
-      ps = ps + ls[a] * wt;
+      ps = ps + ln[a] * wt;
       if (VEC) {
-        ths = ths + lt[a] * wt;
-        phs = phs + lp[a] * wt;
+        ths = ths + ln[n + a] * wt;
+        phs = phs + ln[2 * n + a] * wt;
       }
     }
     size_t o;
@@ -155,9 +172,9 @@ struct StagePlane {
 #define SOLAR_STAGE_ATTR
 #endif
 
-template <int TI, int TJ, int KIND>
-__global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, const double* __restrict__ qin,
-                                                   double* __restrict__ qout, double* __restrict__ qw,
+template <int TI, int TJ, int KIND, typename S>
+__global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, const S* __restrict__ qin,
+                                                   S* __restrict__ qout, S* __restrict__ qw,
                                                    const double* __restrict__ gxp, const double* __restrict__ gyp,
                                                    const double* __restrict__ gxv, const double* __restrict__ gyv,
                                                    double alpha, double beta, double dt, int kchunk, int nkch,
@@ -185,13 +202,13 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t ij = (size_t)j * n + i;
-  const double* f0 = qin + (size_t)(P * 5) * nf;  // rho'
-  const double* f1 = f0 + nf;                     // v_xi
-  const double* f2 = f1 + nf;                     // v_eta
-  const double* f3 = f2 + nf;                     // v_r
-  const double* f4 = f3 + nf;                     // p'
-  double* o0 = qout + (size_t)(P * 5) * nf;
-  double* w0 = qw + (size_t)(P * 5) * nf;
+  const S* f0 = qin + (size_t)(P * 5) * nf;  // rho'
+  const S* f1 = f0 + nf;                     // v_xi
+  const S* f2 = f1 + nf;                     // v_eta
+  const S* f3 = f2 + nf;                     // v_r
+  const S* f4 = f3 + nf;                     // p'
+  S* o0 = qout + (size_t)(P * 5) * nf;
+  S* w0 = qw + (size_t)(P * 5) * nf;
 
   // Halo cells this thread fills: source pointer and plane stride, from the
   // field inside the panel or from the gathered ghost strips beyond its edge.
@@ -201,70 +218,76 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
   const double* gypS = gyp + (size_t)(P * 2) * gy_side;
   const double* gyvS = gyv + (size_t)(P * 2) * gy_side;
   const bool hx = tx < 3 || tx >= TI - 3, hy0 = ty < 3, hy1 = ty >= TJ - 3;
-  const double *hx_p = f4, *hx_v = f1, *hy0_p = f4, *hy0_v = f2, *hy1_p = f4, *hy1_v = f2;
-  size_t hx_s = plane, hy0_s = plane, hy1_s = plane;
+  // Each halo slot: a 32-bit offset inside one plane of its source and
+  // whether the source is a ghost strip (plane stride 3n) or the field
+  // (plane stride n*n); the p' and velocity halos share the offset.
+  int hx_off = 0, hy0_off = 0, hy1_off = 0;
+  bool hx_g = false, hy0_g = false, hy1_g = false;
   int hx_col = 0;
-  if (tx < 3) {
-    const int ii = i0 - 3 + tx;
-    hx_col = tx;
-    if (ii >= 0) {
-      hx_p = f4 + (size_t)j * n + ii; hx_v = f1 + (size_t)j * n + ii;
+  if (hx) {
+    const bool lo = tx < 3;
+    const int ii = lo ? i0 - 3 + tx : i + 3;
+    hx_col = lo ? tx : tx + 6;
+    if (ii >= 0 && ii < n) {
+      hx_off = j * n + ii;
     } else {
-      hx_p = gxpW + (size_t)j * 3 + (-ii - 1); hx_v = gxvW + (size_t)j * 3 + (-ii - 1); hx_s = (size_t)n * 3;
-    }
-  } else if (tx >= TI - 3) {
-    const int ii = i + 3;
-    hx_col = tx + 6;
-    if (ii < n) {
-      hx_p = f4 + (size_t)j * n + ii; hx_v = f1 + (size_t)j * n + ii;
-    } else {
-      hx_p = gxpW + gx_side + (size_t)j * 3 + (ii - n); hx_v = gxvW + gx_side + (size_t)j * 3 + (ii - n);
-      hx_s = (size_t)n * 3;
+      hx_g = true;
+      hx_off = lo ? j * 3 + (-ii - 1) : (int)gx_side + j * 3 + (ii - n);
     }
   }
   if (hy0) {
     const int jj = j0 - 3 + ty;
     if (jj >= 0) {
-      hy0_p = f4 + (size_t)jj * n + i; hy0_v = f2 + (size_t)jj * n + i;
+      hy0_off = jj * n + i;
     } else {
-      hy0_p = gypS + (size_t)(-jj - 1) * n + i; hy0_v = gyvS + (size_t)(-jj - 1) * n + i; hy0_s = (size_t)3 * n;
+      hy0_g = true;
+      hy0_off = (-jj - 1) * n + i;
     }
   }
   if (hy1) {
     const int jj = j + 3;
     if (jj < n) {
-      hy1_p = f4 + (size_t)jj * n + i; hy1_v = f2 + (size_t)jj * n + i;
+      hy1_off = jj * n + i;
     } else {
-      hy1_p = gypS + gy_side + (size_t)(jj - n) * n + i; hy1_v = gyvS + gy_side + (size_t)(jj - n) * n + i;
-      hy1_s = (size_t)3 * n;
+      hy1_g = true;
+      hy1_off = (int)gy_side + (jj - n) * n + i;
     }
   }
 
   auto load_plane = [&](int k, StagePlane& L) {
     const size_t kp = (size_t)k * plane + ij;
-    L.vx = f1[kp];
-    L.ve = f2[kp];
-    L.q1 = f0[kp];
-    L.pc = f4[kp];
-    L.rc = f3[kp];
+    L.vx = static_cast<double>(f1[kp]);
+    L.ve = static_cast<double>(f2[kp]);
+    L.q1 = static_cast<double>(__builtin_nontemporal_load(f0 + kp));
+    // p' and v_r at plane k are the window centre on interior planes; the
+    // three end planes on each side read them here.
+    if (k < 3 || k >= nr - 3) {
+      L.pc = static_cast<double>(f4[kp]);
+      L.rc = static_cast<double>(f3[kp]);
+    } else {
+      L.pc = 0.0;
+      L.rc = 0.0;
+    }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) L.w[f] = (KIND == 0) ? 0.0 : w0[f * nf + kp];
+    for (int f = 0; f < 5; ++f)
+      L.w[f] = (KIND == 0) ? 0.0 : static_cast<double>(__builtin_nontemporal_load(w0 + f * nf + kp));
+    const size_t kf = (size_t)k * plane, kg = (size_t)k * 3 * n;
     if (hx) {
-      L.hpx = hx_p[(size_t)k * hx_s];
-      L.hvx = hx_v[(size_t)k * hx_s];
+      L.hpx = hx_g ? gxpW[kg + hx_off] : static_cast<double>(f4[kf + hx_off]);
+      L.hvx = hx_g ? gxvW[kg + hx_off] : static_cast<double>(f1[kf + hx_off]);
     }
     if (hy0) {
-      L.hpy0 = hy0_p[(size_t)k * hy0_s];
-      L.hve0 = hy0_v[(size_t)k * hy0_s];
+      L.hpy0 = hy0_g ? gypS[kg + hy0_off] : static_cast<double>(f4[kf + hy0_off]);
+      L.hve0 = hy0_g ? gyvS[kg + hy0_off] : static_cast<double>(f2[kf + hy0_off]);
     }
     if (hy1) {
-      L.hpy1 = hy1_p[(size_t)k * hy1_s];
-      L.hve1 = hy1_v[(size_t)k * hy1_s];
+      L.hpy1 = hy1_g ? gypS[kg + hy1_off] : static_cast<double>(f4[kf + hy1_off]);
+      L.hve1 = hy1_g ? gyvS[kg + hy1_off] : static_cast<double>(f2[kf + hy1_off]);
     }
     // plane entering the radial window when the window slides before k
     const int kn = min(k + 3, nr - 1);
-    L.pn = f4[(size_t)kn * plane + ij];
-    L.rn = f3[(size_t)kn * plane + ij];
+    L.pn = static_cast<double>(f4[(size_t)kn * plane + ij]);
+    L.rn = static_cast<double>(f3[(size_t)kn * plane + ij]);
   };
 
   const double Ci = T.ci[i], Dj = T.dj[j];
@@ -275,8 +298,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
   int kb = clampi(k0 - 3, 0, nr - 7);
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
-    pw[q] = f4[(size_t)(kb + q) * plane + ij];
-    rw[q] = f3[(size_t)(kb + q) * plane + ij];
+    pw[q] = static_cast<double>(f4[(size_t)(kb + q) * plane + ij]);
+    rw[q] = static_cast<double>(f3[(size_t)(kb + q) * plane + ij]);
   }
   StagePlane nxt;
   load_plane(k0, nxt);
@@ -295,8 +318,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
       rw[6] = cur.rn;
       kb = want;
     }
+    const bool interior = k >= 3 && k < nr - 3;
+    const double pc = interior ? pw[3] : cur.pc;
+    const double rc = interior ? rw[3] : cur.rc;
     const int bf = k & 1;
-    sp[bf][ty + 3][tx + 3] = cur.pc;
+    sp[bf][ty + 3][tx + 3] = pc;
     svx[bf][ty][tx + 3] = cur.vx;
     sve[bf][ty + 3][tx] = cur.ve;
     if (hx) {
@@ -326,7 +352,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
       dvr = dvr + row.w[q] * rw[q];
       dpr = dpr + row.w[q] * pw[q];
     }
-    const double vx = cur.vx, ve = cur.ve, rc = cur.rc, pc = cur.pc, q1 = cur.q1;
+    const double vx = cur.vx, ve = cur.ve, q1 = cur.q1;
 
     // grad.f90:125-133 and :49-57.
     const double div =
@@ -358,11 +384,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
 
     const size_t x = (size_t)k * plane + ij;
     if (rhs_mode) {
-      o0[x] = r1;
-      o0[nf + x] = r2;
-      o0[2 * nf + x] = r3;
-      o0[3 * nf + x] = r4;
-      o0[4 * nf + x] = r5;
+      o0[x] = static_cast<S>(r1);
+      o0[nf + x] = static_cast<S>(r2);
+      o0[2 * nf + x] = static_cast<S>(r3);
+      o0[3 * nf + x] = static_cast<S>(r4);
+      o0[4 * nf + x] = static_cast<S>(r5);
     } else {
       // timestep.f90:68-69: qw = alpha*qw + dt*rhs ; q = q + beta*qw
       const double rr[5] = {r1, r2, r3, r4, r5};
@@ -370,8 +396,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
         const double wn = alpha * cur.w[f] + dt * rr[f];
-        o0[f * nf + x] = qq[f] + beta * wn;
-        if (KIND != 2) w0[f * nf + x] = wn;
+        __builtin_nontemporal_store(static_cast<S>(qq[f] + beta * wn), o0 + f * nf + x);
+        if (KIND != 2) __builtin_nontemporal_store(static_cast<S>(wn), w0 + f * nf + x);
       }
     }
   }
@@ -384,9 +410,9 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
 // (prefetched into registers one plane ahead), FILTER_XI runs on the in-panel
 // rows, FILTER_ETA on the result, and FILTER_R from an 11-plane register
 // window.
-template <int TI, int TJ>
-__global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __restrict__ qin,
-                                                    double* __restrict__ qout, const double* __restrict__ fgx,
+template <int TI, int TJ, typename S>
+__global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restrict__ qin,
+                                                    S* __restrict__ qout, const double* __restrict__ fgx,
                                                     const double* __restrict__ fgy, int kchunk, int nkch) {
   constexpr int NT = TI * TJ;
   constexpr int NCELL = (TJ + 6) * (TI + 6);
@@ -413,8 +439,8 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
   if (k0 >= nr) return;
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const double* g = qin + (size_t)PF * nf;
-  double* o = qout + (size_t)PF * nf;
+  const S* g = qin + (size_t)PF * nf;
+  S* o = qout + (size_t)PF * nf;
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
   const size_t fstride = (size_t)12 * gx_side;
   const double* gxW = fgx + F * fstride + (size_t)(P * 2) * gx_side;
@@ -423,54 +449,56 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
 
   // filter.f90:652,726-777: the five planes at each end are zero.
   for (int k = k0; k < k1; ++k)
-    if (k < kNstr || k >= nr - kNstr) o[(size_t)k * plane + ij] = 0.0;
+    if (k < kNstr || k >= nr - kNstr) o[(size_t)k * plane + ij] = static_cast<S>(0.0);
   const int a = max(k0, kNstr), bnd = min(k1, nr - kNstr);
   if (a >= bnd) return;
 
   // Tile cells this thread fills, with their source and plane stride.  Corner
   // cells beyond the panel in both directions are never read and stay 0.
   const int tid = ty * TI + tx;
-  const double* cptr[NC];
-  size_t cstr[NC];
+  Src<S> csrc[NC];
   bool cval[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int e = tid + c * NT;
-    cptr[c] = g;
-    cstr[c] = 0;
     cval[c] = false;
     if (e < NCELL) {
       const int rr = e / (TI + 6), cc = e % (TI + 6);
       const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
       const bool jin = gj >= 0 && gj < n, iin = gi >= 0 && gi < n;
       if (jin && iin) {
-        cptr[c] = g + (size_t)gj * n + gi; cstr[c] = plane; cval[c] = true;
+        csrc[c].f = g + (size_t)gj * n + gi; csrc[c].stride = plane; cval[c] = true;
       } else if (jin) {
-        cptr[c] = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
-        cstr[c] = (size_t)n * 3; cval[c] = true;
+        csrc[c].g = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
+        csrc[c].stride = (size_t)n * 3; cval[c] = true;
       } else if (iin) {
-        cptr[c] = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
-        cstr[c] = (size_t)3 * n; cval[c] = true;
+        csrc[c].g = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
+        csrc[c].stride = (size_t)3 * n; cval[c] = true;
       }
     }
   }
-  double nx[NC];
-  auto load_plane = [&](int kk) {
+  // Two planes in flight: nx1 holds plane kk+1, nx2 plane kk+2.
+  double nx1[NC], nx2[NC];
+  auto load_plane = [&](int kk, double (&dst)[NC]) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) nx[c] = cval[c] ? cptr[c][(size_t)kk * cstr[c]] : 0.0;
+    for (int c = 0; c < NC; ++c) dst[c] = cval[c] ? csrc[c].at(kk) : 0.0;
   };
 
   double win[11];
 #pragma unroll
   for (int q = 0; q < 11; ++q) win[q] = 0.0;
 
-  const int kbeg = a - kNstr, kend = bnd + kNstr;
-  load_plane(kbeg);
+  const int kbeg = a - kNstr, kend = bnd + kNstr;  // kend - kbeg >= 11
+  load_plane(kbeg, nx1);
+  load_plane(kbeg + 1, nx2);
   for (int kk = kbeg; kk < kend; ++kk) {
     double cur[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) cur[c] = nx[c];
-    if (kk + 1 < kend) load_plane(kk + 1);
+    for (int c = 0; c < NC; ++c) {
+      cur[c] = nx1[c];
+      nx1[c] = nx2[c];
+    }
+    if (kk + 2 < kend) load_plane(kk + 2, nx2);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int e = tid + c * NT;
@@ -481,15 +509,15 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
     // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
     for (int rr = ty; rr < TJ + 6; rr += TJ) {
       const int gj = j0 - 3 + rr;
-      double v;
+      double xv;
       if (gj >= 0 && gj < n) {
-        v = 0.0;
+        xv = 0.0;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) v = v + T.filt[q] * sq[rr][tx + q];
+        for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
       } else {
-        v = sq[rr][tx + 3];
+        xv = sq[rr][tx + 3];
       }
-      sx[rr][tx] = v;
+      sx[rr][tx] = xv;
     }
     __syncthreads();
     // FILTER_ETA (filter.f90:448-629).
@@ -504,23 +532,24 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const double* __
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < 11; ++q) s = s + T.filtr[q] * win[q];
-      o[(size_t)kout * plane + ij] = s;
+      o[(size_t)kout * plane + ij] = static_cast<S>(s);
     }
   }
 }
 
 // Non-finite scan and v_r sum of squares; acc[0] counts non-finite values,
 // acc[1] accumulates squares.
-__global__ void k_scan(const double* __restrict__ q, size_t total, size_t nf, double* acc, int mode) {
+template <typename S>
+__global__ void k_scan(const S* __restrict__ q, size_t total, size_t nf, double* acc, int mode) {
   double s = 0.0;
   for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
     if (mode == 0) {
-      const double v = q[x];
+      const double v = static_cast<double>(q[x]);
       s += (isfinite(v) ? 0.0 : 1.0);
     } else {
       // v_r of panel p: field 3
       const size_t p = x / nf, r = x % nf;
-      const double v = q[(p * 5 + 3) * nf + r];
+      const double v = static_cast<double>(q[(p * 5 + 3) * nf + r]);
       s += v * v;
     }
   }
@@ -535,11 +564,12 @@ __global__ void k_zero(double* p, int count) {
 
 // Gathers field slots of a staging state buffer into place: dst[(p*5+dfield)]
 // <- src[p] for the ghost test entry points.
-__global__ void k_place(const double* __restrict__ src, double* __restrict__ dst, size_t nf, int dfield) {
+template <typename S>
+__global__ void k_place(const double* __restrict__ src, S* __restrict__ dst, size_t nf, int dfield) {
   const size_t total = nf * 6;
   for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
     const size_t p = x / nf, r = x % nf;
-    dst[(p * 5 + dfield) * nf + r] = src[x];
+    dst[(p * 5 + dfield) * nf + r] = static_cast<S>(src[x]);
   }
 }
 
@@ -555,21 +585,28 @@ int chunks_for(int tiles, int nr, int target_blocks, int min_chunk) {
   return nk;
 }
 
-hipError_t launch_gather(const DevTables& T, const double* q, bool vec, int sfield, int nfields, double* gxs,
-                         double* gys, double* gxv, double* gyv, hipStream_t st) {
+template <typename S>
+static hipError_t launch_gather_t(const DevTables& T, const S* q, bool vec, int sfield, int nfields, double* gxs,
+                                  double* gys, double* gxv, double* gyv, hipStream_t st) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
-  dim3 grid((unsigned)(72 * T.nr), (unsigned)nfields);
-  const int threads = T.n < 256 ? T.n : 256;
-  const size_t lds = (size_t)3 * T.n * sizeof(double);
+  dim3 grid((unsigned)(24 * T.nr), (unsigned)nfields);
+  const size_t lds = (size_t)9 * T.n * sizeof(double);
   if (vec)
-    hipLaunchKernelGGL(k_gather<true>, grid, dim3(threads), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+    hipLaunchKernelGGL((k_gather<true, S>), grid, dim3(256), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
   else
-    hipLaunchKernelGGL(k_gather<false>, grid, dim3(threads), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+    hipLaunchKernelGGL((k_gather<false, S>), grid, dim3(256), lds, st, T, q, sfield, gxs, gys, gxv, gyv,
+                       field_stride);
   return hipGetLastError();
 }
 
-template <int TI, int TJ>
-static hipError_t launch_stage_t(const DevTables& T, const double* qin, double* qout, double* qw, const double* gxp,
+hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
+                         double* gys, double* gxv, double* gyv, hipStream_t st) {
+  if (f32) return launch_gather_t(T, static_cast<const float*>(q), vec, sfield, nfields, gxs, gys, gxv, gyv, st);
+  return launch_gather_t(T, static_cast<const double*>(q), vec, sfield, nfields, gxs, gys, gxv, gyv, st);
+}
+
+template <int TI, int TJ, typename S>
+static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
@@ -578,56 +615,112 @@ static hipError_t launch_stage_t(const DevTables& T, const double* qin, double* 
   dim3 grid(tiles * nkch), block(TI, TJ);
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   if (rhs_mode || stage == 0)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 0>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
+    hipLaunchKernelGGL((k_stage<TI, TJ, 0, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
                        kchunk, nkch, rhs_mode);
   else if (stage < 4)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 1>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
+    hipLaunchKernelGGL((k_stage<TI, TJ, 1, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
                        kchunk, nkch, rhs_mode);
   else
-    hipLaunchKernelGGL((k_stage<TI, TJ, 2>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
+    hipLaunchKernelGGL((k_stage<TI, TJ, 2, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
                        kchunk, nkch, rhs_mode);
   return hipGetLastError();
 }
 
-hipError_t launch_stage(const DevTables& T, const double* qin, double* qout, double* qw, const double* gxp,
-                        const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                        hipStream_t st, const LaunchShape& ls) {
-  switch (pick_ti(T.n)) {
-    case 64: return launch_stage_t<64, 4>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
-    case 32: return launch_stage_t<32, 8>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
-    default: return launch_stage_t<16, 16>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+// Tile shapes (TI x TJ) compiled for the stage and filter kernels; the
+// default is the measured best for the widest TI dividing n.  A requested
+// shape that does not divide n falls back to the default.
+static bool fits(int n, int ti, int tj) { return ti > 0 && tj > 0 && n % ti == 0 && n % tj == 0; }
+
+template <typename S>
+static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
+                                 const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
+                                 hipStream_t st, const LaunchShape& ls) {
+  int ti = ls.stage_ti, tj = ls.stage_tj;
+  if (!fits(T.n, ti, tj)) {  // measured best: 64x8 (n%64==0), 32x8, 16x16
+    ti = pick_ti(T.n);
+    tj = ti == 64 ? 8 : 256 / ti;
   }
+#define SOLAR_STAGE_CASE(A, B) \
+  if (ti == A && tj == B)      \
+    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+  SOLAR_STAGE_CASE(64, 8)
+  SOLAR_STAGE_CASE(64, 4)
+  SOLAR_STAGE_CASE(32, 8)
+  SOLAR_STAGE_CASE(16, 16)
+  SOLAR_STAGE_CASE(32, 16)
+  SOLAR_STAGE_CASE(32, 4)
+#undef SOLAR_STAGE_CASE
+  ti = pick_ti(T.n);
+  if (ti == 64) return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+  if (ti == 32) return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
 }
 
-template <int TI, int TJ>
-static hipError_t launch_filter_t(const DevTables& T, const double* qin, double* qout, const double* fgx,
-                                  const double* fgy, hipStream_t st, const LaunchShape& ls) {
+hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
+                        const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
+                        hipStream_t st, const LaunchShape& ls) {
+  if (f32)
+    return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
+                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+  return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
+                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+}
+
+template <int TI, int TJ, typename S>
+static hipError_t launch_filter_t(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
+                                  hipStream_t st, const LaunchShape& ls) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 30;
   const int nkch = ls.filter_chunks > 0 ? ls.filter_chunks : chunks_for(tiles, T.nr, 2048, 48);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
-  hipLaunchKernelGGL((k_filter<TI, TJ>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
+  hipLaunchKernelGGL((k_filter<TI, TJ, S>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
   return hipGetLastError();
 }
 
-hipError_t launch_filter(const DevTables& T, const double* qin, double* qout, const double* fgx, const double* fgy,
-                         hipStream_t st, const LaunchShape& ls) {
-  switch (pick_ti(T.n)) {
-    case 64: return launch_filter_t<64, 4>(T, qin, qout, fgx, fgy, st, ls);
-    case 32: return launch_filter_t<32, 8>(T, qin, qout, fgx, fgy, st, ls);
-    default: return launch_filter_t<16, 16>(T, qin, qout, fgx, fgy, st, ls);
+template <typename S>
+static hipError_t launch_filter_s(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
+                                  hipStream_t st, const LaunchShape& ls) {
+  int ti = ls.filter_ti, tj = ls.filter_tj;
+  if (!fits(T.n, ti, tj)) {  // measured best: 32x16 (n%32==0), 16x16
+    ti = T.n % 32 == 0 ? 32 : 16;
+    tj = 16;
   }
+#define SOLAR_FILTER_CASE(A, B) \
+  if (ti == A && tj == B) return launch_filter_t<A, B, S>(T, qin, qout, fgx, fgy, st, ls);
+  SOLAR_FILTER_CASE(32, 16)
+  SOLAR_FILTER_CASE(16, 16)
+  SOLAR_FILTER_CASE(64, 4)
+  SOLAR_FILTER_CASE(32, 8)
+  SOLAR_FILTER_CASE(64, 8)
+  SOLAR_FILTER_CASE(64, 16)
+#undef SOLAR_FILTER_CASE
+  if (T.n % 32 == 0) return launch_filter_t<32, 16, S>(T, qin, qout, fgx, fgy, st, ls);
+  return launch_filter_t<16, 16, S>(T, qin, qout, fgx, fgy, st, ls);
 }
 
-hipError_t launch_scan(const double* q, size_t nf, double* acc, int mode, hipStream_t st) {
+hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
+                         const double* fgy, hipStream_t st, const LaunchShape& ls) {
+  if (f32)
+    return launch_filter_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), fgx, fgy, st, ls);
+  return launch_filter_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), fgx, fgy, st, ls);
+}
+
+hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, acc, 2);
   const size_t total = mode == 0 ? nf * 30 : nf * 6;
-  hipLaunchKernelGGL(k_scan, dim3(1024), dim3(256), 0, st, q, total, nf, acc, mode);
+  if (f32)
+    hipLaunchKernelGGL(k_scan<float>, dim3(1024), dim3(256), 0, st, static_cast<const float*>(q), total, nf, acc, mode);
+  else
+    hipLaunchKernelGGL(k_scan<double>, dim3(1024), dim3(256), 0, st, static_cast<const double*>(q), total, nf, acc,
+                       mode);
   return hipGetLastError();
 }
 
-hipError_t launch_place(const double* src, double* dst, size_t nf, int dfield, hipStream_t st) {
-  hipLaunchKernelGGL(k_place, dim3(1024), dim3(256), 0, st, src, dst, nf, dfield);
+hipError_t launch_place(const double* src, void* dst, bool f32, size_t nf, int dfield, hipStream_t st) {
+  if (f32)
+    hipLaunchKernelGGL(k_place<float>, dim3(1024), dim3(256), 0, st, src, static_cast<float*>(dst), nf, dfield);
+  else
+    hipLaunchKernelGGL(k_place<double>, dim3(1024), dim3(256), 0, st, src, static_cast<double*>(dst), nf, dfield);
   return hipGetLastError();
 }
 

@@ -28,16 +28,19 @@ struct DevTables {
 struct LaunchShape {
   int stage_chunks = 0;
   int filter_chunks = 0;
+  int stage_ti = 0, stage_tj = 0;
+  int filter_ti = 0, filter_tj = 0;
 };
 
-hipError_t launch_gather(const DevTables& T, const double* q, bool vec, int sfield, int nfields, double* gxs,
+// State pointers are float* when f32 is set (FP32-state variant), double* otherwise.
+hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
                          double* gys, double* gxv, double* gyv, hipStream_t st);
-hipError_t launch_stage(const DevTables& T, const double* qin, double* qout, double* qw, const double* gxp,
+hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls);
-hipError_t launch_filter(const DevTables& T, const double* qin, double* qout, const double* fgx, const double* fgy,
-                         hipStream_t st, const LaunchShape& ls);
-hipError_t launch_scan(const double* q, size_t nf, double* acc, int mode, hipStream_t st);
-hipError_t launch_place(const double* src, double* dst, size_t nf, int dfield, hipStream_t st);
+hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
+                         const double* fgy, hipStream_t st, const LaunchShape& ls);
+hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st);
+hipError_t launch_place(const double* src, void* dst, bool f32, size_t nf, int dfield, hipStream_t st);
 
 }  // namespace solar

@@ -22,6 +22,9 @@ _lib = None
 EXPORTS = [
     ("solar_init", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _D, _D, _D, _D,
                                   ctypes.c_int, ctypes.c_double]),
+    ("solar_init_ex", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _D, _D, _D, _D,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    ("solar_state_bits", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     ("solar_finalize", ctypes.c_int, [ctypes.c_void_p]),
     ("solar_set_state", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_get_state", ctypes.c_int, [ctypes.c_void_p, _D]),
@@ -130,13 +133,15 @@ class Solver:
     Arrays are numpy float64 of shape (6, 5, nr, n, n) (= Fortran q(n,n,nr,5,6)).
     """
 
-    def __init__(self, n, nr, gr, c2, rho0, g0, sponge_len, a_sponge=0.1):
+    def __init__(self, n, nr, gr, c2, rho0, g0, sponge_len, a_sponge=0.1, state_bits=64):
         self.lib = load_library()
         self.n, self.nr = n, nr
         self.shape = (6, 5, nr, n, n)
+        self.state_bits = state_bits
         h = ctypes.c_void_p()
-        _check(self.lib.solar_init(ctypes.byref(h), n, nr, _p(_f64(gr)), _p(_f64(c2)), _p(_f64(rho0)),
-                                   _p(_f64(g0)), int(sponge_len), float(a_sponge)), "solar_init")
+        _check(self.lib.solar_init_ex(ctypes.byref(h), n, nr, _p(_f64(gr)), _p(_f64(c2)), _p(_f64(rho0)),
+                                      _p(_f64(g0)), int(sponge_len), float(a_sponge), int(state_bits)),
+               "solar_init")
         self.h = h
         self.time = 0.0
 
