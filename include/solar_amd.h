@@ -128,6 +128,45 @@ int solar_kernel_times(solar_ctx* ctx, double* out);
 /* Bytes of device memory held by the handle. */
 int solar_device_bytes(solar_ctx* ctx, size_t* bytes);
 
+/* Sensitivities (discrete adjoint).  No counterpart in the reference, which
+ * has no adjoint: these serve inversion and data-assimilation callers of the
+ * same RK4 step.  All need a 64-bit state handle. ----------------------- */
+
+/* Replaces the radial c^2(r) of solar_init by a full field c2(n,n,nr,6) in
+ * COMPUTE_RHS's pressure equation: the reference's c2(i,j,k) is already a 3-D
+ * array (compute_rhs_acoustic.f90:134) that SOLAR_DATA happens to fill
+ * radially.  rho0 and g0 stay radial.  NULL restores the radial profile.
+ * The host array is copied. */
+int solar_set_c2_field(solar_ctx* ctx, const double* c2);
+
+/* Receivers: nrec points of the p' field given as (panel, k, j, i) 0-based
+ * quadruples in pkji[4*nrec]. */
+int solar_set_receivers(solar_ctx* ctx, int nrec, const int* pkji);
+
+/* nsteps RK4 steps from the current state, sampling p' at the receivers after
+ * every step: seis[nsteps][nrec]. */
+int solar_record(solar_ctx* ctx, int nsteps, double dt, double* seis);
+
+/* Transposes of the step's linear parts on host arrays q(n,n,nr,5,6):
+ *   solar_adj_rhs    out = L^T w,  L = SPONGE . BC_RHS . COMPUTE_RHS
+ *   solar_adj_filter out = F^T o,  F = FILTER_EXP
+ *   solar_adj_step   out = S^T lam, S = one RK4 step (timestep.f90:41-86)
+ * Each is exact to rounding: <L x, w> = <x, L^T w>.  Do not touch the state. */
+int solar_adj_rhs(solar_ctx* ctx, const double* w, double* out);
+int solar_adj_filter(solar_ctx* ctx, const double* o, double* out);
+int solar_adj_step(solar_ctx* ctx, double dt, const double* lam, double* out);
+
+/* Misfit J = sum over steps and receivers of (p'(rec) - data)^2 after nsteps
+ * RK4 steps from q0, and its gradient grad(n,n,nr,6) = dJ/dc2 with respect
+ * to the c^2 field in use (radial profile or solar_set_c2_field).  The
+ * forward pass keeps one state checkpoint per checkpoint_every steps; the
+ * reverse pass recomputes each segment, storing div(v) of every stage, and
+ * runs the adjoint steps over it.  data[nsteps][nrec]; seis (may be NULL)
+ * receives the modelled traces.  Leaves the final forward state of the
+ * first segment in the handle's state. */
+int solar_misfit_gradient(solar_ctx* ctx, const double* q0, int nsteps, double dt, const double* data,
+                          int checkpoint_every, double* J, double* grad, double* seis);
+
 /* Description of the last error on this thread ("" if none). */
 const char* solar_last_error(void);
 

@@ -43,6 +43,7 @@ def _load():
         fn.restype = None
         fn.argtypes = [v] + [_D] * nargs
     lib.orc_bc_rhs.argtypes = [v, _D]
+    lib.orc_set_c2_field.argtypes = [v, _D]
     lib.orc_ddr.argtypes = [v, _D, _D, ctypes.c_long]
     lib.orc_rk4.argtypes = [v, _D, ctypes.c_int, ctypes.c_double, _D]
     lib.orc_convert.argtypes = [v, ctypes.c_int, _D, _D, _D, _D]
@@ -81,6 +82,13 @@ class Oracle:
 
     def _out(self, shape=None):
         return np.zeros(shape or self.shape)
+
+    def set_c2_field(self, c2):
+        """c2(i,j,k) per panel, shape (6, nr, n, n); None restores the radial profile."""
+        if c2 is None:
+            self.lib.orc_set_c2_field(self.h, None)
+        else:
+            self.lib.orc_set_c2_field(self.h, _p(_c(c2)))
 
     def compute_rhs(self, q):
         out = self._out()

@@ -74,6 +74,7 @@ typedef struct {
   double *interpoints;                           /* N*3 : [l*N + i] */
   double *interpolant;                           /* 9*N*3 : [(l*N + i)*9 + iter+4] */
   double *c2, *rho0, *g0;                        /* R radial background profiles */
+  double* c2f; /* optional c2(i,j,k) per panel, (N,N,R,6); NULL = radial c2 */
   /* transform.f90:138-155 ghost-coordinate tables */
   double Xm[4], Xp[4], Ym[4], Yp[4], Cm[4], Cp[4], Dm[4], Dp[4];
   double *global_C, *global_D; /* N */
@@ -210,7 +211,22 @@ void orc_destroy(orc_t* o) {
                   o->gchi, o->drdchi, o->c2, o->rho0, o->g0, o->mdelta, o->sqrtd, o->XY, o->CD,
                   o->interpoints, o->interpolant};
   for (size_t a = 0; a < sizeof(ps) / sizeof(ps[0]); a++) free(ps[a]);
+  free(o->c2f);
   free(o);
+}
+
+/* The reference's c2 is the 3-D array c2(mx,my,mr) (compute_rhs_acoustic.f90:134,
+ * ic_acoustic_wave.f90:257-271 fills it from a radial profile).  A non-NULL
+ * field c2f(N,N,R,6) replaces the radial profile in COMPUTE_RHS. */
+void orc_set_c2_field(orc_t* o, const double* c2f) {
+  const size_t cnt = (size_t)6 * o->N * o->N * o->R;
+  if (!c2f) {
+    free(o->c2f);
+    o->c2f = NULL;
+    return;
+  }
+  if (!o->c2f) o->c2f = (double*)calloc(cnt, sizeof(double));
+  memcpy(o->c2f, c2f, cnt * sizeof(double));
 }
 
 /* Table export for the table-parity test. */
@@ -786,7 +802,7 @@ void orc_compute_rhs(const orc_t* o, const double* q, double* rhs) {
     for (int k = 0; k < R; k++)
       for (size_t c = 0; c < (size_t)N * N; c++) {
         const size_t x = (size_t)k * N * N + c;
-        const double rho0 = o->rho0[k], g0 = o->g0[k], c2 = o->c2[k];
+        const double rho0 = o->rho0[k], g0 = o->g0[k], c2 = o->c2f ? o->c2f[z * nf + x] : o->c2[k];
         const double dv = div[z * nf + x];
         double r1 = -(rho0 * dv);
         r1 = r1 - qz[3 * nf + x] * drho0dr[k];

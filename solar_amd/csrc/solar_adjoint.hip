@@ -1,0 +1,584 @@
+// Discrete adjoint of the acoustic RK4 step, FP64, gfx950.
+//
+// The forward step is linear in the state q: one stage computes
+//   r = L(y) = SPONGE(BC_RHS(COMPUTE_RHS(y)))      (compute_rhs_acoustic.f90,
+//   w = alpha_s w + dt r,  y = y + beta_s w         bc_acoustic_wave.f90,
+//                                                  sponge.f90, timestep.f90:64-69)
+// and the step ends with q = FILTER_EXP(y) (timestep.f90:73-75).  These
+// kernels apply the exact transposes:
+//   k_adj_stage    one reverse RK stage: Ybar += dt * L^T(W), the next W, and
+//                  the sound-speed gradient  c2bar += dt * (-rho0 div) * W5.
+//                  L^T = transposed DRP stencils (zero extension; the radial
+//                  Tam-Dong rows transposed) applied to the transposed
+//                  pointwise terms; edge tiles emit the adjoints of the ghost
+//                  values they would have read.
+//   k_adj_scatter  the transpose of the panel gather: ghost adjoints pushed
+//                  through the receiver-basis transform, the 9-point Lagrange
+//                  weights and the (xi,eta)->(theta,phi) conversion into the
+//                  sender panel's near-edge rows (atomic adds: strips of one
+//                  panel meet at its corners).
+//   k_adj_filter_r / k_adj_filter_xe   FILTER_R^T, then FILTER_ETA^T and
+//                  FILTER_XI^T with the ghost adjoints of both passes (the
+//                  eta-pass ghosts come from the unfiltered field,
+//                  filter.f90:25-28,43).
+#include <hip/hip_runtime.h>
+
+#include "solar_adjoint.hpp"
+
+namespace solar {
+
+namespace {
+
+__device__ __forceinline__ int clampi_a(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ int strip_index_a(int e, int l, int a, int n) {
+  switch (e) {
+    case kS: return l * n + a;
+    case kN: return (n - 1 - l) * n + a;
+    case kW: return a * n + l;
+    default: return a * n + (n - 1 - l);
+  }
+}
+
+// Base plane of the DDR stencil of plane k (derivative.f90:656-754).
+__device__ __forceinline__ int ddr_base(int k, int nr) { return k < 3 ? 0 : (k >= nr - 3 ? nr - 7 : k - 3); }
+
+struct AdjU {
+  double uxv, uyv, urv, uxp, uyp, urp;  // inputs of the transposed DDX/DDY/DDR
+  double divb, b1, b4, b5;              // divergence adjoint and BC-masked W
+};
+
+// Transposed pointwise part of COMPUTE_RHS at one point (BC_RHS^T applied to
+// W first: BC zeroes the end planes of the RHS).
+__device__ __forceinline__ void adj_point(const DevTables& T, const RadialRow& row, int k, int i, int j,
+                                          double c2rho0, double w1, double w2, double w3, double w4, double w5,
+                                          AdjU& u) {
+  const bool bc = (k == 0 || k == T.nr - 1);
+  const double b1 = bc ? 0.0 : w1, b2 = bc ? 0.0 : w2, b3 = bc ? 0.0 : w3, b4 = bc ? 0.0 : w4,
+               b5 = bc ? 0.0 : w5;
+  const size_t ij = (size_t)j * T.n + i;
+  const double Ci = T.ci[i], Dj = T.dj[j], dc = T.dc[ij], xy = T.xy[ij];
+  // r1 = -rho0 div - v_r drho0dr ; r5 = -c2rho0 div + rho0 g0 v_r
+  const double divb = -row.rho0 * b1 - c2rho0 * b5;
+  // r2..r4 = -grad/rho0
+  const double gxb = -b2 / row.rho0, geb = -b3 / row.rho0, grb = -b4 / row.rho0;
+  // div = inv_r*(dc*(Dx vxi/C + Dy veta/D) - tdx vxi - tde veta) + Dr vr + 2 vr/r
+  u.uxv = row.inv_r * dc / Ci * divb;
+  u.uyv = row.inv_r * dc / Dj * divb;
+  u.urv = divb;
+  // gxi = inv_r*(D Dx p + XY Dy p/D), geta = inv_r*(C Dy p + XY Dx p/C), gr = Dr p
+  u.uxp = row.inv_r * (Dj * gxb + xy / Ci * geb);
+  u.uyp = row.inv_r * (xy / Dj * gxb + Ci * geb);
+  u.urp = grb;
+  u.divb = divb;
+  u.b1 = b1;
+  u.b4 = b4;
+  u.b5 = b5;
+}
+
+__device__ __forceinline__ double c2rho0_at(const DevTables& T, const RadialRow& row, const double* c2f, int P,
+                                           int k, size_t ij) {
+  return c2f ? c2f[(size_t)P * T.n * T.n * T.nr + (size_t)k * T.n * T.n + ij] * row.rho0 : row.c2rho0;
+}
+
+}  // namespace
+
+// One reverse RK stage of the adjoint (or L^T alone in RHS mode).
+//   W    : W^(s) = the adjoint of the stage's 2N accumulator (5 fields)
+//   Yin  : Ybar^(s) (5 fields); Yout = Ybar^(s-1) = Yin + dt L^T W (may alias Yin)
+//   Wout : W^(s-1) = alpha*W + beta*Yout when mode & kAdjWriteW
+//   divs : div(v) of the forward stage input, for the gradient (mode & kAdjGrad)
+//   gc2  : c^2 gradient accumulator (n,n,nr,6)
+//   gxa_*, gya_* : dt-scaled ghost adjoints written by edge tiles
+// In RHS mode (mode & kAdjRhsOnly) Yout = L^T W and dt must be 1.
+template <int TI, int TJ>
+__global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double* __restrict__ W,
+                                                       const double* Yin, double* Yout,  // may alias
+                                                       double* __restrict__ Wout, const double* __restrict__ divs,
+                                                       double* __restrict__ gc2, const double* __restrict__ c2f,
+                                                       double* __restrict__ gxa_p, double* __restrict__ gya_p,
+                                                       double* __restrict__ gxa_v, double* __restrict__ gya_v,
+                                                       double dt, double alpha, double beta, int kchunk, int nkch,
+                                                       int mode) {
+  __shared__ double sxv[TJ][TI + 6], sxp[TJ][TI + 6];
+  __shared__ double syv[TJ + 6][TI], syp[TJ + 6][TI];
+
+  const int n = T.n, nr = T.nr;
+  const int nti = n / TI, ntj = n / TJ;
+  int b = blockIdx.x;
+  const int bi = b % nti;
+  b /= nti;
+  const int bj = b % ntj;
+  b /= ntj;
+  const int P = b % 6;
+  const int ch = b / 6;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = bi * TI, j0 = bj * TJ;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = ch * kchunk;
+  const int k1 = min(k0 + kchunk, nr);
+  if (k0 >= nr) return;
+
+  const size_t plane = (size_t)n * n, nf = plane * nr;
+  const size_t ij = (size_t)j * n + i;
+  const double* w0 = W + (size_t)(P * 5) * nf;
+  auto wat = [&](int f, int k, size_t cell) { return w0[(size_t)f * nf + (size_t)k * plane + cell]; };
+
+  // Radial windows of the transposed-DDR inputs over planes k-3..k+3, plus the
+  // three closure rows at each shell end (their stencils reach planes 0..6 and
+  // nr-7..nr-1, derivative.f90:704-716,742-754).
+  double rv[7], rp[7], rv_lo[3], rp_lo[3], rv_hi[3], rp_hi[3];
+  auto radial_u = [&](int k, double& uv, double& up) {
+    if (k < 0 || k >= nr) {
+      uv = 0.0;
+      up = 0.0;
+      return;
+    }
+    const RadialRow& row = T.rad[k];
+    AdjU u;
+    adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), wat(0, k, ij), 0.0, 0.0, wat(3, k, ij),
+              wat(4, k, ij), u);
+    uv = u.urv;
+    up = u.urp;
+  };
+#pragma unroll
+  for (int q = 0; q < 7; ++q) radial_u(k0 - 3 + q, rv[q], rp[q]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    rv_lo[q] = rp_lo[q] = rv_hi[q] = rp_hi[q] = 0.0;
+    if (k0 <= 6) radial_u(q, rv_lo[q], rp_lo[q]);
+    if (k1 >= nr - 7) radial_u(nr - 3 + q, rv_hi[q], rp_hi[q]);
+  }
+
+  const bool hx_lo = tx < 3, hx_hi = tx >= TI - 3, hy_lo = ty < 3, hy_hi = ty >= TJ - 3;
+  const double sig_dt = (mode & kAdjRhsOnly) ? 1.0 : dt;
+
+  for (int k = k0; k < k1; ++k) {
+    if (k > k0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        rv[q] = rv[q + 1];
+        rp[q] = rp[q + 1];
+      }
+      radial_u(k + 3, rv[6], rp[6]);
+    }
+    const RadialRow& row = T.rad[k];
+    const size_t kp = (size_t)k * plane;
+    // own point
+    const double w1 = w0[kp + ij], w2 = w0[nf + kp + ij], w3 = w0[2 * nf + kp + ij], w4 = w0[3 * nf + kp + ij],
+                 w5 = w0[4 * nf + kp + ij];
+    AdjU u;
+    adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), w1, w2, w3, w4, w5, u);
+    sxv[ty][tx + 3] = u.uxv;
+    sxp[ty][tx + 3] = u.uxp;
+    syv[ty + 3][tx] = u.uyv;
+    syp[ty + 3][tx] = u.uyp;
+    // halo cells (zero beyond the panel edge: the transposed stencil has no ghosts)
+    if (hx_lo || hx_hi) {
+      const int ii = hx_lo ? i0 - 3 + tx : i + 3;
+      const int col = hx_lo ? tx : tx + 6;
+      double a = 0.0, c = 0.0;
+      if (ii >= 0 && ii < n) {
+        const size_t cell = (size_t)j * n + ii;
+        AdjU h;
+        adj_point(T, row, k, ii, j, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
+                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
+        a = h.uxv;
+        c = h.uxp;
+      }
+      sxv[ty][col] = a;
+      sxp[ty][col] = c;
+    }
+    if (hy_lo) {
+      const int jj = j0 - 3 + ty;
+      double a = 0.0, c = 0.0;
+      if (jj >= 0) {
+        const size_t cell = (size_t)jj * n + i;
+        AdjU h;
+        adj_point(T, row, k, i, jj, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
+                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
+        a = h.uyv;
+        c = h.uyp;
+      }
+      syv[ty][tx] = a;
+      syp[ty][tx] = c;
+    }
+    if (hy_hi) {
+      const int jj = j + 3;
+      double a = 0.0, c = 0.0;
+      if (jj < n) {
+        const size_t cell = (size_t)jj * n + i;
+        AdjU h;
+        adj_point(T, row, k, i, jj, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
+                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
+        a = h.uyv;
+        c = h.uyp;
+      }
+      syv[ty + 6][tx] = a;
+      syp[ty + 6][tx] = c;
+    }
+    __syncthreads();
+
+    // Transposed stencils: (D^T u)(i) = sum_q c[6-q] u(i-3+q).
+    double dxv = 0.0, dxp = 0.0, dyv = 0.0, dyp = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      dxv += T.fcx[6 - q] * sxv[ty][tx + q];
+      dxp += T.fcx[6 - q] * sxp[ty][tx + q];
+      dyv += T.fcy[6 - q] * syv[ty + q][tx];
+      dyp += T.fcy[6 - q] * syp[ty + q][tx];
+    }
+    // Transposed radial stencil: every row kk whose DDR stencil reads plane k.
+    double drv = 0.0, drp = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      const int kk = k - 3 + q;
+      if (kk >= 0 && kk < nr) {
+        const int t = k - ddr_base(kk, nr);
+        if (t >= 0 && t <= 6) {
+          const double wt = T.rad[kk].w[t];
+          drv += wt * rv[q];
+          drp += wt * rp[q];
+        }
+      }
+    }
+    if (k <= 6) {
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk)
+        if (kk < k - 3) {
+          const double wt = T.rad[kk].w[k];
+          drv += wt * rv_lo[kk];
+          drp += wt * rp_lo[kk];
+        }
+    }
+    if (k >= nr - 7) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int kk = nr - 3 + q;
+        if (kk > k + 3) {
+          const double wt = T.rad[kk].w[k - (nr - 7)];
+          drv += wt * rv_hi[q];
+          drp += wt * rp_hi[q];
+        }
+      }
+    }
+
+    const double sg = row.sigma;
+    const double tdx = T.tdx[ij], tde = T.tde[ij];
+    double lt[5];
+    lt[0] = -(row.g0 / row.rho0) * u.b4 - sg * w1;
+    lt[1] = -row.inv_r * tdx * u.divb + dxv;
+    lt[2] = -row.inv_r * tde * u.divb + dyv;
+    lt[3] = -row.drho0dr * u.b1 + row.rho0g0 * u.b5 + (2.0 / row.r) * u.divb - sg * w4 + drv;
+    lt[4] = -sg * w5 + dxp + dyp + drp;
+
+    const size_t x = kp + ij;
+    double* yo = Yout + (size_t)(P * 5) * nf;
+    if (mode & kAdjRhsOnly) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) yo[(size_t)f * nf + x] = lt[f];
+    } else {
+      const double* yi = Yin + (size_t)(P * 5) * nf;
+      double* wo = Wout + (size_t)(P * 5) * nf;
+      const double wv[5] = {w1, w2, w3, w4, w5};
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double yn = yi[(size_t)f * nf + x] + dt * lt[f];
+        yo[(size_t)f * nf + x] = yn;
+        if (mode & kAdjWriteW) wo[(size_t)f * nf + x] = alpha * wv[f] + beta * yn;
+      }
+      if (mode & kAdjGrad) {
+        // r5 = -c^2 rho0 div + ...  ->  d r5 / d c^2 = -rho0 div (zero on the BC planes)
+        const size_t g = (size_t)P * nf + x;
+        gc2[g] += dt * (-row.rho0 * divs[g]) * u.b5;
+      }
+    }
+
+    // Ghost adjoints of edge tiles: the receiver's DDX/DDY read ghost layer
+    // l at i = -l (W), n-1+l (E), j = -l (S), n-1+l (N) with tap 3-l-i, etc.
+    const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
+    if (bi == 0 && tx < 3) {
+      const int l = tx + 1;
+      double gp = 0.0, gv = 0.0;
+      for (int ii = 0; ii <= 3 - l; ++ii) {
+        gp += T.fcx[3 - l - ii] * sxp[ty][3 + ii];
+        gv += T.fcx[3 - l - ii] * sxv[ty][3 + ii];
+      }
+      const size_t o = (size_t)(P * 2) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
+      gxa_p[o] = sig_dt * gp;
+      gxa_v[o] = sig_dt * gv;
+    }
+    if (bi == nti - 1 && tx >= TI - 3) {
+      const int l = tx - (TI - 3) + 1;
+      double gp = 0.0, gv = 0.0;
+      for (int ii = n - 4 + l; ii <= n - 1; ++ii) {
+        gp += T.fcx[n + 2 + l - ii] * sxp[ty][3 + ii - i0];
+        gv += T.fcx[n + 2 + l - ii] * sxv[ty][3 + ii - i0];
+      }
+      const size_t o = (size_t)(P * 2 + 1) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
+      gxa_p[o] = sig_dt * gp;
+      gxa_v[o] = sig_dt * gv;
+    }
+    if (bj == 0 && ty < 3) {
+      const int l = ty + 1;
+      double gp = 0.0, gv = 0.0;
+      for (int jj = 0; jj <= 3 - l; ++jj) {
+        gp += T.fcy[3 - l - jj] * syp[3 + jj][tx];
+        gv += T.fcy[3 - l - jj] * syv[3 + jj][tx];
+      }
+      const size_t o = (size_t)(P * 2) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
+      gya_p[o] = sig_dt * gp;
+      gya_v[o] = sig_dt * gv;
+    }
+    if (bj == ntj - 1 && ty >= TJ - 3) {
+      const int l = ty - (TJ - 3) + 1;
+      double gp = 0.0, gv = 0.0;
+      for (int jj = n - 4 + l; jj <= n - 1; ++jj) {
+        gp += T.fcy[n + 2 + l - jj] * syp[3 + jj - j0][tx];
+        gv += T.fcy[n + 2 + l - jj] * syv[3 + jj - j0][tx];
+      }
+      const size_t o = (size_t)(P * 2 + 1) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
+      gya_p[o] = sig_dt * gp;
+      gya_v[o] = sig_dt * gv;
+    }
+    __syncthreads();
+  }
+}
+
+// Transpose of k_gather: one block per (receiver panel P, edge e, plane k).
+// Adds the ghost adjoints to the sender's rows of Y (and beta times them to
+// Wn when given).  Scalars: field sfield (+blockIdx.y); vectors: fields 1, 2.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* __restrict__ gxs,
+                                                     const double* __restrict__ gys, const double* __restrict__ gxv,
+                                                     const double* __restrict__ gyv, double* Y, double* Wn,
+                                                     double beta, int sfield, long long field_stride_in) {
+  const int n = T.n, nr = T.nr;
+  const int edge = blockIdx.x;
+  const int k = edge / 24;
+  const int P = (edge % 24) / 4;
+  const int e = edge % 4;
+  const int fld = blockIdx.y;
+  const Route rt = T.route[P * 4 + e];
+  const int s = rt.sender, se = rt.sender_edge;
+  const size_t plane = (size_t)n * n, nf = plane * nr;
+  const size_t fo = (size_t)fld * field_stride_in;
+  const int ctype = s < 4 ? 0 : s - 3;
+  const double* cv = T.conv + (size_t)ctype * plane * 6;
+  double* ys = Y + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
+  double* ws = Wn ? Wn + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane : nullptr;
+  double* y1 = Y + ((size_t)(s * 5 + 1) * nr + k) * plane;
+  double* y2 = y1 + nf;
+  double* w1 = Wn ? Wn + ((size_t)(s * 5 + 1) * nr + k) * plane : nullptr;
+  double* w2 = Wn ? w1 + nf : nullptr;
+  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+    const int l = t / n + 1, m = t % n;
+    size_t o;
+    if (e < 2)
+      o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
+    else
+      o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
+    const double gs = (e < 2) ? gxs[o] : gys[o];
+    double th = 0.0, ph = 0.0;
+    const int mp = rt.reverse ? n - 1 - m : m;
+    if (VEC) {
+      const double gv = (e < 2) ? gxv[o] : gyv[o];
+      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
+      th = ab[0] * gv;
+      ph = ab[1] * gv;
+    }
+    const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
+    for (int tp = 0; tp < 9; ++tp) {
+      const double wt = w[tp];
+      if (wt == 0.0) continue;
+      const int a = clampi_a(mp - 4 + tp, 0, n - 1);
+      const int idx = strip_index_a(se, l, a, n);
+      atomicAdd(ys + idx, wt * gs);
+      if (ws) atomicAdd(ws + idx, beta * wt * gs);
+      if (VEC) {
+        const double* c = cv + (size_t)idx * 6;
+        const double tb = wt * th, pb = wt * ph;
+        const double ax = c[0] * c[1] * tb + c[3] * c[4] * pb;
+        const double ay = c[0] * c[2] * tb + c[3] * c[5] * pb;
+        atomicAdd(y1 + idx, ax);
+        atomicAdd(y2 + idx, ay);
+        if (w1) {
+          atomicAdd(w1 + idx, beta * ax);
+          atomicAdd(w2 + idx, beta * ay);
+        }
+      }
+    }
+  }
+}
+
+// FILTER_R^T per point: out(k') = sum over k in [5, nr-6], |k-k'| <= 5 of
+// filt_r[k'-k+5] in(k)  (filter.f90:682-693; the end planes carry no value).
+__global__ void k_adj_filter_r(DevTables T, const double* __restrict__ in, double* __restrict__ out) {
+  const int n = T.n, nr = T.nr;
+  const size_t plane = (size_t)n * n, nf = plane * nr, total = nf * 30;
+  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
+    const size_t fo = x / nf * nf, r = x % nf;
+    const int k = (int)(r / plane);
+    const size_t c = r % plane;
+    double s = 0.0;
+    for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
+      s += T.filtr[k - kk + 5] * in[fo + (size_t)kk * plane + c];
+    out[x] = s;
+  }
+}
+
+// FILTER_ETA^T then FILTER_XI^T on one plane of one field (zero extension),
+// with the ghost adjoints of both passes for edge tiles (scaled by scale).
+template <int TI, int TJ>
+__global__ __launch_bounds__(TI* TJ) void k_adj_filter_xe(DevTables T, const double* __restrict__ bin,
+                                                           double* __restrict__ out, double* __restrict__ fgxa,
+                                                           double* __restrict__ fgya) {
+  __shared__ double sb[TJ + 6][TI + 6];
+  __shared__ double sa[TJ][TI + 6];
+  const int n = T.n, nr = T.nr;
+  const int nti = n / TI, ntj = n / TJ;
+  int b = blockIdx.x;
+  const int bi = b % nti;
+  b /= nti;
+  const int bj = b % ntj;
+  b /= ntj;
+  const int PF = b % 30;
+  const int k = b / 30;
+  const int P = PF / 5, F = PF % 5;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i0 = bi * TI, j0 = bj * TJ;
+  const int i = i0 + tx, j = j0 + ty;
+  const size_t plane = (size_t)n * n, nf = plane * nr;
+  const double* g = bin + (size_t)PF * nf + (size_t)k * plane;
+  for (int e = ty * TI + tx; e < (TJ + 6) * (TI + 6); e += TI * TJ) {
+    const int rr = e / (TI + 6), cc = e % (TI + 6);
+    const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
+    sb[rr][cc] = (gj >= 0 && gj < n && gi >= 0 && gi < n) ? g[(size_t)gj * n + gi] : 0.0;
+  }
+  __syncthreads();
+  // abar(row ty, column c) = sum_q fe[6-q] bbar(row ty-3+q)
+  for (int cc = tx; cc < TI + 6; cc += TI) {
+    double a = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) a += T.filt[6 - q] * sb[ty + q][cc];
+    sa[ty][cc] = a;
+  }
+  __syncthreads();
+  double qb = 0.0;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) qb += T.filt[6 - q] * sa[ty][tx + q];
+  out[(size_t)PF * nf + (size_t)k * plane + (size_t)j * n + i] = qb;
+
+  const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
+  const size_t fst = (size_t)12 * gx_side;
+  if (bj == 0 && ty < 3) {  // eta-pass ghosts below the S edge
+    const int l = ty + 1;
+    double s = 0.0;
+    for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * sb[3 + jj][tx + 3];
+    fgya[F * fst + (size_t)(P * 2) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i] = s;
+  }
+  if (bj == ntj - 1 && ty >= TJ - 3) {
+    const int l = ty - (TJ - 3) + 1;
+    double s = 0.0;
+    for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * sb[3 + jj - j0][tx + 3];
+    fgya[F * fst + (size_t)(P * 2 + 1) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i] = s;
+  }
+  if (bi == 0 && tx < 3) {  // xi-pass ghosts west of the W edge
+    const int l = tx + 1;
+    double s = 0.0;
+    for (int ii = 0; ii <= 3 - l; ++ii) s += T.filt[3 - l - ii] * sa[ty][3 + ii];
+    fgxa[F * fst + (size_t)(P * 2) * gx_side + ((size_t)k * n + j) * 3 + (l - 1)] = s;
+  }
+  if (bi == nti - 1 && tx >= TI - 3) {
+    const int l = tx - (TI - 3) + 1;
+    double s = 0.0;
+    for (int ii = n - 4 + l; ii <= n - 1; ++ii) s += T.filt[n + 2 + l - ii] * sa[ty][3 + ii - i0];
+    fgxa[F * fst + (size_t)(P * 2 + 1) * gx_side + ((size_t)k * n + j) * 3 + (l - 1)] = s;
+  }
+}
+
+// y = a*x (whole 5-field state).
+__global__ void k_scale(const double* __restrict__ x, double* __restrict__ y, double a, size_t total) {
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    y[t] = a * x[t];
+}
+
+// Receiver sampling and adjoint sources.
+__global__ void k_sample(const double* __restrict__ q, const long long* __restrict__ idx, int nrec,
+                         double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrec) out[r] = q[idx[r]];
+}
+__global__ void k_inject(double* __restrict__ lam, const long long* __restrict__ idx, int nrec,
+                         const double* __restrict__ seis, const double* __restrict__ data) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nrec) lam[idx[r]] += 2.0 * (seis[r] - data[r]);
+}
+
+// ------------------------------------------------------------------ launchers
+
+static int adj_ti(int n) { return (n % 32 == 0) ? 32 : 16; }
+
+template <int TI, int TJ>
+static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, hipStream_t st) {
+  const int tiles = (T.n / TI) * (T.n / TJ) * 6;
+  int nkch = (2048 + tiles - 1) / tiles;
+  const int maxk = T.nr / 32 > 0 ? T.nr / 32 : 1;
+  if (nkch > maxk) nkch = maxk;
+  if (nkch < 1) nkch = 1;
+  const int kchunk = (T.nr + nkch - 1) / nkch;
+  hipLaunchKernelGGL((k_adj_stage<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout, a.Wout,
+                     a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, kchunk, nkch,
+                     a.mode);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream_t st) {
+  if (adj_ti(T.n) == 32) return launch_adj_stage_t<32, 8>(T, a, st);
+  return launch_adj_stage_t<16, 16>(T, a, st);
+}
+
+hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
+                              const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
+                              hipStream_t st) {
+  const long long field_stride = 12LL * T.nr * T.n * 3;
+  dim3 grid((unsigned)(24 * T.nr), (unsigned)nfields);
+  if (vec)
+    hipLaunchKernelGGL(k_adj_scatter<true>, grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield,
+                       field_stride);
+  else
+    hipLaunchKernelGGL(k_adj_scatter<false>, grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield,
+                       field_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_filter(const DevTables& T, const double* in, double* tmp, double* out, double* fgxa,
+                             double* fgya, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_filter_r, dim3(4096), dim3(256), 0, st, T, in, tmp);
+  if (adj_ti(T.n) == 32) {
+    const int tiles = (T.n / 32) * (T.n / 8) * 30 * T.nr;
+    hipLaunchKernelGGL((k_adj_filter_xe<32, 8>), dim3(tiles), dim3(32, 8), 0, st, T, tmp, out, fgxa, fgya);
+  } else {
+    const int tiles = (T.n / 16) * (T.n / 16) * 30 * T.nr;
+    hipLaunchKernelGGL((k_adj_filter_xe<16, 16>), dim3(tiles), dim3(16, 16), 0, st, T, tmp, out, fgxa, fgya);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scale(const double* x, double* y, double a, size_t total, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(4096), dim3(256), 0, st, x, y, a, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(const double* q, const long long* idx, int nrec, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sample, dim3((nrec + 63) / 64), dim3(64), 0, st, q, idx, nrec, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_inject(double* lam, const long long* idx, int nrec, const double* seis, const double* data,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_inject, dim3((nrec + 63) / 64), dim3(64), 0, st, lam, idx, nrec, seis, data);
+  return hipGetLastError();
+}
+
+}  // namespace solar

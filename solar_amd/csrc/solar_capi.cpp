@@ -3,6 +3,7 @@
 // q(n,n,nr,5,6) and its work buffers on one HIP device and one stream.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "../../include/solar_amd.h"
+#include "solar_adjoint.hpp"
 #include "solar_kernels.hpp"
 #include "solar_tables.hpp"
 
@@ -62,6 +64,13 @@ struct solar_ctx {
   std::vector<hipEvent_t> ev;
   double ktime[5] = {0, 0, 0, 0, 0};
   size_t bytes = 0;
+  // sensitivity / adjoint state, allocated on first use
+  double* c2f = nullptr;  // optional 3-D c^2 (n,n,nr,6); points at c2f_buf when set
+  double* c2f_buf = nullptr;
+  double* c2f_graph = nullptr;
+  double *wb[2] = {nullptr, nullptr}, *yb = nullptr, *tmp = nullptr, *lam = nullptr, *gc2 = nullptr;
+  long long* rec = nullptr;  // receiver indices into the state (field p')
+  int nrec = 0, rec_cap = 0;
 };
 
 namespace {
@@ -101,7 +110,7 @@ void release(solar_ctx* c) {
 // Enqueues one RK4 step (timestep.f90:61-82) on the handle's stream.  The
 // state lives in q[0] before and after: five stages ping-pong q[0] <-> q[1],
 // the filter brings it back.  With `timed`, events bracket every launch.
-int enqueue_step(solar_ctx* c, double dt, bool timed) {
+int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = nullptr) {
   const DevTables& T = c->dt;
   void* cur = c->q[0];
   void* nxt = c->q[1];
@@ -111,7 +120,7 @@ int enqueue_step(solar_ctx* c, double dt, bool timed) {
     HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
     HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
-                               c->shape));
+                               c->shape, c->c2f, divp ? divp[s] : nullptr));
     void* t = cur;
     cur = nxt;
     nxt = t;
@@ -142,6 +151,7 @@ int build_graph(solar_ctx* c, double dt) {
   c->graph = g;
   HIPCHK(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
   c->graph_dt = dt;
+  c->c2f_graph = c->c2f;
   return SOLAR_OK;
 }
 
@@ -346,7 +356,7 @@ int solar_rk4_async(solar_ctx* c, int nsteps, double dt) {
     return SOLAR_OK;
   }
   if (c->use_graph) {
-    if (!c->graph_exec || c->graph_dt != dt) {
+    if (!c->graph_exec || c->graph_dt != dt || c->c2f_graph != c->c2f) {
       int rc = build_graph(c, dt);
       if (rc) return rc;
     }
@@ -385,7 +395,7 @@ int solar_compute_rhs(solar_ctx* c, const double* q, double* q_rhs, int flags) {
   HIPCHK(solar::launch_gather(c->dt, c->q[1], c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
   const int mode = 4 | (flags & (SOLAR_RHS_BC | SOLAR_RHS_SPONGE));
   HIPCHK(solar::launch_stage(c->dt, c->q[1], c->qw, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, -1, 0.0, mode,
-                             c->stream, c->shape));
+                             c->stream, c->shape, c->c2f));
   return download_state(c, q_rhs, c->qw, nq);
 }
 
@@ -501,4 +511,252 @@ int solar_host_tables(int n, int nr, const double* gr, const double* c2, const d
   return SOLAR_OK;
 }
 
+
+// ------------------------------------------------------------ sensitivities
+
+static int adj_alloc(solar_ctx* c) {
+  if (c->yb) return SOLAR_OK;
+  if (c->f32) return fail(SOLAR_ERR_ARG, "adjoint: needs a 64-bit state handle");
+  const size_t nq = c->nf * 30;
+  int rc;
+  if ((rc = dev_alloc(c, &c->wb[0], nq)) || (rc = dev_alloc(c, &c->wb[1], nq)) || (rc = dev_alloc(c, &c->yb, nq)) ||
+      (rc = dev_alloc(c, &c->tmp, nq)) || (rc = dev_alloc(c, &c->lam, nq)) || (rc = dev_alloc(c, &c->gc2, c->nf * 6)))
+    return rc;
+  return SOLAR_OK;
+}
+
+static int upload_f64(solar_ctx* c, double* dst, const double* src, size_t count) {
+  HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return SOLAR_OK;
+}
+static int download_f64(solar_ctx* c, double* dst, const double* src, size_t count) {
+  HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+// F^T then the five reverse stages of one step: out = S^T in.  With divp, the
+// c^2 gradient of this step is added to c->gc2 (divp[s] = div(v) of the
+// forward stage s input).
+static int enqueue_adj_step(solar_ctx* c, double dt, const double* in, double* out, double* const* divp) {
+  const solar::DevTables& T = c->dt;
+  const size_t nq = c->nf * 30;
+  // FILTER_EXP^T: out = F^T in  (tmp holds FILTER_R^T in)
+  HIPCHK(solar::launch_adj_filter(T, in, c->tmp, out, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_scatter(T, false, c->fgx, c->fgy, nullptr, nullptr, out, nullptr, 0.0, 0, 5, c->stream));
+  // W^(5) = beta_5 Ybar^(5)
+  HIPCHK(solar::launch_scale(out, c->wb[0], solar::kBetaRk4[4], nq, c->stream));
+  int cur = 0;
+  for (int s = 5; s >= 1; --s) {
+    solar::AdjStageArgs a;
+    a.W = c->wb[cur];
+    a.Yin = out;
+    a.Yout = out;
+    a.Wout = s > 1 ? c->wb[1 - cur] : nullptr;
+    a.dt = dt;
+    a.alpha = solar::kAlphaRk4[s - 1];
+    a.beta = s > 1 ? solar::kBetaRk4[s - 2] : 0.0;
+    a.mode = (s > 1 ? solar::kAdjWriteW : 0) | (divp ? solar::kAdjGrad : 0);
+    a.divs = divp ? divp[s - 1] : nullptr;
+    a.gc2 = c->gc2;
+    a.c2f = c->c2f;
+    a.gxa_p = c->gxp;
+    a.gya_p = c->gyp;
+    a.gxa_v = c->gxv;
+    a.gya_v = c->gyv;
+    HIPCHK(solar::launch_adj_stage(T, a, c->stream));
+    HIPCHK(solar::launch_adj_scatter(T, true, c->gxp, c->gyp, c->gxv, c->gyv, out, a.Wout, a.beta, 4, 1, c->stream));
+    cur = 1 - cur;
+  }
+  return SOLAR_OK;
+}
+
+int solar_set_c2_field(solar_ctx* c, const double* c2) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_set_c2_field: null handle");
+  if (!c2) {
+    c->c2f = nullptr;
+    return SOLAR_OK;
+  }
+  double* buf = c->c2f_buf;
+  if (!buf) {
+    int rc = dev_alloc(c, &buf, c->nf * 6);
+    if (rc) return rc;
+    c->c2f_buf = buf;
+  }
+  HIPCHK(hipMemcpyAsync(buf, c2, c->nf * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->c2f = buf;
+  return SOLAR_OK;
+}
+
+int solar_set_receivers(solar_ctx* c, int nrec, const int* pkji) {
+  if (!c || nrec < 0 || (nrec > 0 && !pkji)) return fail(SOLAR_ERR_ARG, "solar_set_receivers: invalid argument");
+  std::vector<long long> idx(nrec);
+  for (int r = 0; r < nrec; ++r) {
+    const int p = pkji[4 * r], k = pkji[4 * r + 1], j = pkji[4 * r + 2], i = pkji[4 * r + 3];
+    if (p < 0 || p >= 6 || k < 0 || k >= c->nr || j < 0 || j >= c->n || i < 0 || i >= c->n)
+      return fail(SOLAR_ERR_ARG, "solar_set_receivers: receiver " + std::to_string(r) + " outside the grid");
+    idx[r] = ((static_cast<long long>(p) * 5 + 4) * c->nr + k) * c->n * c->n + static_cast<long long>(j) * c->n + i;
+  }
+  if (nrec > c->rec_cap) {
+    long long* d = nullptr;
+    int rc = dev_alloc(c, &d, static_cast<size_t>(nrec));
+    if (rc) return rc;
+    c->rec = d;
+    c->rec_cap = nrec;
+  }
+  c->nrec = nrec;
+  if (nrec) HIPCHK(hipMemcpyAsync(c->rec, idx.data(), nrec * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SOLAR_OK;
+}
+
+int solar_record(solar_ctx* c, int nsteps, double dt, double* seis) {
+  if (!c || nsteps < 0 || (nsteps && !seis)) return fail(SOLAR_ERR_ARG, "solar_record: invalid argument");
+  if (c->f32) return fail(SOLAR_ERR_ARG, "solar_record: needs a 64-bit state handle");
+  const size_t count = static_cast<size_t>(nsteps) * c->nrec;
+  if (count == 0) return solar_rk4(c, nsteps, dt, nullptr);
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, count * sizeof(double));
+  if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("solar_record: hipMalloc: ") + hipGetErrorString(e));
+  double* d = static_cast<double*>(v);
+  int rc = SOLAR_OK;
+  for (int s = 0; s < nsteps && rc == SOLAR_OK; ++s) {
+    rc = solar_rk4_async(c, 1, dt);
+    if (rc == SOLAR_OK) {
+      e = solar::launch_sample(static_cast<const double*>(c->q[0]), c->rec, c->nrec, d + (size_t)s * c->nrec,
+                               c->stream);
+      if (e != hipSuccess) rc = fail(SOLAR_ERR_HIP, std::string("solar_record: ") + hipGetErrorString(e));
+    }
+  }
+  if (rc == SOLAR_OK) rc = download_f64(c, seis, d, count);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(v);
+  return rc;
+}
+
+int solar_adj_rhs(solar_ctx* c, const double* w, double* out) {
+  if (!c || !w || !out) return fail(SOLAR_ERR_ARG, "solar_adj_rhs: null argument");
+  int rc = adj_alloc(c);
+  if (rc) return rc;
+  const size_t nq = c->nf * 30;
+  if ((rc = upload_f64(c, c->wb[0], w, nq))) return rc;
+  solar::AdjStageArgs a;
+  a.W = c->wb[0];
+  a.Yout = c->yb;
+  a.c2f = c->c2f;
+  a.mode = solar::kAdjRhsOnly;
+  a.gxa_p = c->gxp;
+  a.gya_p = c->gyp;
+  a.gxa_v = c->gxv;
+  a.gya_v = c->gyv;
+  HIPCHK(solar::launch_adj_stage(c->dt, a, c->stream));
+  HIPCHK(solar::launch_adj_scatter(c->dt, true, c->gxp, c->gyp, c->gxv, c->gyv, c->yb, nullptr, 0.0, 4, 1, c->stream));
+  return download_f64(c, out, c->yb, nq);
+}
+
+int solar_adj_filter(solar_ctx* c, const double* o, double* out) {
+  if (!c || !o || !out) return fail(SOLAR_ERR_ARG, "solar_adj_filter: null argument");
+  int rc = adj_alloc(c);
+  if (rc) return rc;
+  const size_t nq = c->nf * 30;
+  if ((rc = upload_f64(c, c->lam, o, nq))) return rc;
+  HIPCHK(solar::launch_adj_filter(c->dt, c->lam, c->tmp, c->yb, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_scatter(c->dt, false, c->fgx, c->fgy, nullptr, nullptr, c->yb, nullptr, 0.0, 0, 5, c->stream));
+  return download_f64(c, out, c->yb, nq);
+}
+
+int solar_adj_step(solar_ctx* c, double dt, const double* lam, double* out) {
+  if (!c || !lam || !out) return fail(SOLAR_ERR_ARG, "solar_adj_step: null argument");
+  int rc = adj_alloc(c);
+  if (rc) return rc;
+  const size_t nq = c->nf * 30;
+  if ((rc = upload_f64(c, c->lam, lam, nq))) return rc;
+  if ((rc = enqueue_adj_step(c, dt, c->lam, c->yb, nullptr))) return rc;
+  return download_f64(c, out, c->yb, nq);
+}
+
+int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt, const double* data,
+                          int checkpoint_every, double* J, double* grad, double* seis_out) {
+  if (!c || !q0 || !data || !J || !grad || nsteps < 1 || checkpoint_every < 1)
+    return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: invalid argument");
+  if (c->nrec < 1) return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: no receivers set");
+  int rc = adj_alloc(c);
+  if (rc) return rc;
+  const size_t nq = c->nf * 30, nf6 = c->nf * 6;
+  const int seg = checkpoint_every < nsteps ? checkpoint_every : nsteps;
+  const int nck = (nsteps + seg - 1) / seg;
+  const size_t need = (static_cast<size_t>(nck) * nq + static_cast<size_t>(seg) * 5 * nf6 +
+                       2 * static_cast<size_t>(nsteps) * c->nrec) * sizeof(double);
+  size_t freeb = 0, totalb = 0;
+  HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  if (need + (size_t(1) << 30) > freeb)
+    return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: needs " + std::to_string(need >> 20) + " MiB of checkpoints, " +
+                                   std::to_string(freeb >> 20) + " MiB free; raise checkpoint_every");
+  double *ck = nullptr, *divs = nullptr, *seis = nullptr, *dat = nullptr;
+  std::vector<void*> mine;
+  auto grab = [&](double** p, size_t count) -> int {
+    void* v = nullptr;
+    hipError_t e = hipMalloc(&v, count * sizeof(double));
+    if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    mine.push_back(v);
+    *p = static_cast<double*>(v);
+    return SOLAR_OK;
+  };
+  auto cleanup = [&]() {
+    for (void* v : mine) (void)hipFree(v);
+  };
+  if ((rc = grab(&ck, static_cast<size_t>(nck) * nq)) || (rc = grab(&divs, static_cast<size_t>(seg) * 5 * nf6)) ||
+      (rc = grab(&seis, static_cast<size_t>(nsteps) * c->nrec)) ||
+      (rc = grab(&dat, static_cast<size_t>(nsteps) * c->nrec))) {
+    cleanup();
+    return rc;
+  }
+  auto run = [&]() -> int {
+    int r;
+    if ((r = upload_f64(c, dat, data, static_cast<size_t>(nsteps) * c->nrec))) return r;
+    if ((r = upload_state(c, c->q[0], q0, nq))) return r;
+    // forward pass: checkpoints and receiver traces
+    for (int n = 0; n < nsteps; ++n) {
+      if (n % seg == 0)
+        HIPCHK(hipMemcpyAsync(ck + static_cast<size_t>(n / seg) * nq, c->q[0], nq * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+      if ((r = solar_rk4_async(c, 1, dt))) return r;
+      HIPCHK(solar::launch_sample(static_cast<const double*>(c->q[0]), c->rec, c->nrec,
+                                  seis + static_cast<size_t>(n) * c->nrec, c->stream));
+    }
+    std::vector<double> sh(static_cast<size_t>(nsteps) * c->nrec);
+    if ((r = download_f64(c, sh.data(), seis, sh.size()))) return r;
+    double jj = 0.0;
+    for (size_t x = 0; x < sh.size(); ++x) jj += (sh[x] - data[x]) * (sh[x] - data[x]);
+    *J = jj;
+    if (seis_out) std::memcpy(seis_out, sh.data(), sh.size() * sizeof(double));
+    // reverse pass, one checkpoint segment at a time
+    HIPCHK(hipMemsetAsync(c->lam, 0, nq * sizeof(double), c->stream));
+    HIPCHK(hipMemsetAsync(c->gc2, 0, nf6 * sizeof(double), c->stream));
+    double* lam = c->lam;
+    double* other = c->yb;
+    std::vector<double*> dp(static_cast<size_t>(seg) * 5);
+    for (size_t x = 0; x < dp.size(); ++x) dp[x] = divs + x * nf6;
+    for (int g = nck - 1; g >= 0; --g) {
+      const int n0 = g * seg, n1 = std::min(nsteps, n0 + seg);
+      HIPCHK(hipMemcpyAsync(c->q[0], ck + static_cast<size_t>(g) * nq, nq * sizeof(double), hipMemcpyDeviceToDevice,
+                            c->stream));
+      for (int m = n0; m < n1; ++m)
+        if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - n0) * 5]))) return r;
+      for (int n = n1 - 1; n >= n0; --n) {
+        HIPCHK(solar::launch_inject(lam, c->rec, c->nrec, seis + static_cast<size_t>(n) * c->nrec,
+                                    dat + static_cast<size_t>(n) * c->nrec, c->stream));
+        // lambda_n = S_n^T lambda_{n+1}: reads `lam`, writes `other`, then they swap
+        if ((r = enqueue_adj_step(c, dt, lam, other, &dp[static_cast<size_t>(n - n0) * 5]))) return r;
+        std::swap(lam, other);
+      }
+    }
+    return download_f64(c, grad, c->gc2, nf6);
+  };
+  rc = run();
+  (void)hipStreamSynchronize(c->stream);
+  cleanup();
+  return rc;
+}
 }  // extern "C"

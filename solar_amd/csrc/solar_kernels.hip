@@ -178,7 +178,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
                                                    const double* __restrict__ gxp, const double* __restrict__ gyp,
                                                    const double* __restrict__ gxv, const double* __restrict__ gyv,
                                                    double alpha, double beta, double dt, int kchunk, int nkch,
-                                                   int rhs_mode) {
+                                                   int rhs_mode, const double* __restrict__ c2f,
+                                                   double* __restrict__ divout) {
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
   __shared__ double sve[2][TJ + 6][TI];
@@ -367,7 +368,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
     double r3 = -(get / row.rho0);
     double r4 = -(dpr / row.rho0);
     r4 = r4 - (q1 * row.g0) / row.rho0;
-    double r5 = -(row.c2rho0 * div);
+    // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
+    const double c2rho0 = c2f ? c2f[(size_t)P * nf + (size_t)k * plane + ij] * row.rho0 : row.c2rho0;
+    if (divout) divout[(size_t)P * nf + (size_t)k * plane + ij] = div;
+    double r5 = -(c2rho0 * div);
     r5 = r5 + row.rho0g0 * rc;
 
     const bool do_bc = rhs_mode ? (rhs_mode & 1) : true;
@@ -608,7 +612,8 @@ hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, 
 template <int TI, int TJ, typename S>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
-                                 int rhs_mode, hipStream_t st, const LaunchShape& ls) {
+                                 int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
+                                 double* divout) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 2048, 32);
   const int kchunk = (T.nr + nkch - 1) / nkch;
@@ -616,13 +621,13 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   if (rhs_mode || stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode);
+                       kchunk, nkch, rhs_mode, c2f, divout);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode);
+                       kchunk, nkch, rhs_mode, c2f, divout);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode);
+                       kchunk, nkch, rhs_mode, c2f, divout);
   return hipGetLastError();
 }
 
@@ -634,7 +639,7 @@ static bool fits(int n, int ti, int tj) { return ti > 0 && tj > 0 && n % ti == 0
 template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                                 hipStream_t st, const LaunchShape& ls) {
+                                 hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout) {
   int ti = ls.stage_ti, tj = ls.stage_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 64x8 (n%64==0), 32x8, 16x16
     ti = pick_ti(T.n);
@@ -642,7 +647,7 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   }
 #define SOLAR_STAGE_CASE(A, B) \
   if (ti == A && tj == B)      \
-    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
   SOLAR_STAGE_CASE(64, 8)
   SOLAR_STAGE_CASE(64, 4)
   SOLAR_STAGE_CASE(32, 8)
@@ -651,19 +656,21 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   SOLAR_STAGE_CASE(32, 4)
 #undef SOLAR_STAGE_CASE
   ti = pick_ti(T.n);
-  if (ti == 64) return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
-  if (ti == 32) return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
-  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+  if (ti == 64)
+    return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+  if (ti == 32)
+    return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
 }
 
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                        hipStream_t st, const LaunchShape& ls) {
+                        hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout) {
   if (f32)
     return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
-                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
   return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
-                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls);
+                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
 }
 
 template <int TI, int TJ, typename S>

@@ -35,9 +35,12 @@ struct LaunchShape {
 // State pointers are float* when f32 is set (FP32-state variant), double* otherwise.
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
                          double* gys, double* gxv, double* gyv, hipStream_t st);
+// c2f (optional): a 3-D c^2 field (n,n,nr,6) used instead of the radial
+// profile; divout (optional): receives div(v) of the stage input (n,n,nr,6).
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                        hipStream_t st, const LaunchShape& ls);
+                        hipStream_t st, const LaunchShape& ls, const double* c2f = nullptr,
+                        double* divout = nullptr);
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
                          const double* fgy, hipStream_t st, const LaunchShape& ls);
 hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st);

@@ -41,6 +41,14 @@ EXPORTS = [
     ("solar_norm_vr", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_kernel_times", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("solar_set_c2_field", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_set_receivers", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    ("solar_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]),
+    ("solar_adj_rhs", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("solar_adj_filter", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
+    ("solar_adj_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _D, _D]),
+    ("solar_misfit_gradient", ctypes.c_int, [ctypes.c_void_p, _D, ctypes.c_int, ctypes.c_double, _D, ctypes.c_int,
+                                             _D, _D, _D]),
     ("solar_last_error", ctypes.c_char_p, []),
     ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                          _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -244,3 +252,57 @@ class Solver:
         v = ctypes.c_size_t()
         _check(self.lib.solar_device_bytes(self.h, ctypes.byref(v)), "solar_device_bytes")
         return v.value
+
+    # ---------------------------------------------------------- sensitivities
+
+    def set_c2_field(self, c2):
+        """Full c^2 field, shape (6, nr, n, n) (= Fortran c2(n,n,nr,6)); None restores the radial profile."""
+        if c2 is None:
+            _check(self.lib.solar_set_c2_field(self.h, None), "solar_set_c2_field")
+            return
+        c2 = _f64(c2)
+        if c2.size != 6 * self.nr * self.n * self.n:
+            raise SolarError(f"c2 field of {c2.size} elements, expected shape {(6, self.nr, self.n, self.n)}")
+        _check(self.lib.solar_set_c2_field(self.h, _p(c2)), "solar_set_c2_field")
+
+    def set_receivers(self, pkji):
+        """Receivers of p' at (panel, k, j, i), 0-based; array of shape (nrec, 4)."""
+        a = np.ascontiguousarray(pkji, dtype=np.int32).reshape(-1, 4)
+        self.nrec = a.shape[0]
+        _check(self.lib.solar_set_receivers(self.h, a.shape[0], a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+               "solar_set_receivers")
+
+    def record(self, nsteps, dt):
+        """nsteps RK4 steps, p' at the receivers after each: array (nsteps, nrec)."""
+        out = np.zeros((int(nsteps), getattr(self, "nrec", 0)))
+        _check(self.lib.solar_record(self.h, int(nsteps), float(dt), _p(out)), "solar_record")
+        return out
+
+    def adj_rhs(self, w):
+        out = np.empty(self.shape)
+        _check(self.lib.solar_adj_rhs(self.h, _p(self._arr(w)), _p(out)), "solar_adj_rhs")
+        return out
+
+    def adj_filter(self, o):
+        out = np.empty(self.shape)
+        _check(self.lib.solar_adj_filter(self.h, _p(self._arr(o)), _p(out)), "solar_adj_filter")
+        return out
+
+    def adj_step(self, lam, dt):
+        out = np.empty(self.shape)
+        _check(self.lib.solar_adj_step(self.h, float(dt), _p(self._arr(lam)), _p(out)), "solar_adj_step")
+        return out
+
+    def misfit_gradient(self, q0, nsteps, dt, data, checkpoint_every=50):
+        """(J, dJ/dc2 of shape (6, nr, n, n), modelled traces (nsteps, nrec))."""
+        data = _f64(data)
+        nrec = getattr(self, "nrec", 0)
+        if data.size != int(nsteps) * nrec:
+            raise SolarError(f"data of {data.size} values, expected ({nsteps}, {nrec})")
+        J = ctypes.c_double()
+        grad = np.zeros((6, self.nr, self.n, self.n))
+        seis = np.zeros((int(nsteps), nrec))
+        _check(self.lib.solar_misfit_gradient(self.h, _p(self._arr(q0)), int(nsteps), float(dt), _p(data),
+                                              int(checkpoint_every), ctypes.cast(ctypes.byref(J), _D), _p(grad), _p(seis)),
+               "solar_misfit_gradient")
+        return J.value, grad, seis

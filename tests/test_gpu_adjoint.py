@@ -1,0 +1,164 @@
+"""Discrete adjoint of the RK4 step (sensitivities; the reference has none).
+
+Pinned by what the mathematics fixes rather than by a second implementation:
+  - transpose identities <A x, y> = <x, A^T y> for A = L (one right-hand side
+    with BC_RHS and SPONGE), F (FILTER_EXP) and S (one RK4 step), on random
+    inputs, radial and 3-D c^2 fields, both tile shapes;
+  - the misfit gradient dJ/dc^2 against central finite differences of J along
+    random directions, and its independence of the checkpoint interval.
+The forward path with a 3-D c^2 field is checked bit for bit against the CPU
+restatement, whose c2(i,j,k) array mirrors compute_rhs_acoustic.f90:134.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from solar_amd import RHS_BC, RHS_SPONGE, Config, Solver, background, generate_ic
+
+pytestmark = pytest.mark.gpu
+
+# (n, nr, background kind): n = 32 takes the 32x8 adjoint tiles, 16 and 48 the 16x16 ones
+CASES = [(16, 24, 1), (32, 40, 1), (48, 32, 0)]
+
+
+def _setup(n, nr, kind, steps=1):
+    cfg = Config("adj", n, nr, kind, 0, steps)
+    bg = background(kind, nr)
+    return cfg, bg, cfg.dt(bg[0], bg[1])
+
+
+def _c2_field(bg, n, nr, seed):
+    """Positive 3-D c^2: the radial profile times a random +-10% modulation."""
+    rng = np.random.default_rng(seed)
+    return bg[1][None, :, None, None] * (1.0 + 0.1 * rng.uniform(-1, 1, (6, nr, n, n)))
+
+
+def _transpose_gap(ax, y, x, aty):
+    """|<Ax,y> - <x,A^T y>| relative to |Ax||y| (rounding of FP64 sums is ~1e-16 of that)."""
+    lhs, rhs = float(np.vdot(ax, y)), float(np.vdot(x, aty))
+    return abs(lhs - rhs) / (np.linalg.norm(ax) * np.linalg.norm(y)), lhs, rhs
+
+
+@pytest.mark.parametrize("n,nr,kind", CASES)
+@pytest.mark.parametrize("field", [False, True])
+def test_rhs_transpose(n, nr, kind, field):
+    cfg, bg, _ = _setup(n, nr, kind)
+    rng = np.random.default_rng(n * 100 + nr + field)
+    x = rng.standard_normal((6, 5, nr, n, n))
+    w = rng.standard_normal((6, 5, nr, n, n))
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        if field:
+            s.set_c2_field(_c2_field(bg, n, nr, 7))
+        lx = s.compute_rhs(x, RHS_BC | RHS_SPONGE)
+        ltw = s.adj_rhs(w)
+    gap, a, b = _transpose_gap(lx, w, x, ltw)
+    print(f"[adj] L n={n} nr={nr} field={field}: <Lx,w>={a:.15e} <x,LTw>={b:.15e} gap={gap:.2e}", flush=True)
+    assert gap < 1e-13
+
+
+@pytest.mark.parametrize("n,nr,kind", CASES)
+def test_filter_transpose(n, nr, kind):
+    cfg, bg, _ = _setup(n, nr, kind)
+    rng = np.random.default_rng(n + 3 * nr)
+    x = rng.standard_normal((6, 5, nr, n, n))
+    o = rng.standard_normal((6, 5, nr, n, n))
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        fx = s.filter_exp(x)
+        fto = s.adj_filter(o)
+    gap, a, b = _transpose_gap(fx, o, x, fto)
+    print(f"[adj] F n={n} nr={nr}: gap={gap:.2e}", flush=True)
+    assert gap < 1e-13
+
+
+@pytest.mark.parametrize("n,nr,kind", CASES)
+def test_step_transpose(n, nr, kind):
+    cfg, bg, dt = _setup(n, nr, kind)
+    rng = np.random.default_rng(5 * n + nr)
+    x = rng.standard_normal((6, 5, nr, n, n))
+    lam = rng.standard_normal((6, 5, nr, n, n))
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_c2_field(_c2_field(bg, n, nr, 9))
+        s.set_state(x)
+        s.rk4(1, dt)
+        sx = s.get_state()
+        stl = s.adj_step(lam, dt)
+    gap, a, b = _transpose_gap(sx, lam, x, stl)
+    print(f"[adj] S n={n} nr={nr}: gap={gap:.2e}", flush=True)
+    assert gap < 1e-13
+
+
+def test_c2_field_forward_matches_oracle_bitwise():
+    n, nr, kind = 16, 24, 1
+    cfg, bg, dt = _setup(n, nr, kind)
+    q0, _ = generate_ic(1, n, nr, bg[0], bg[1], 321, [1e-3, 1e-6, 20.0])
+    c2f = _c2_field(bg, n, nr, 11)
+    o = oracle.Oracle(n, nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    o.set_c2_field(c2f)
+    want, _ = o.rk4(q0, 5, dt)
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_c2_field(c2f)
+        s.set_state(q0)
+        s.rk4(5, dt)
+        got = s.get_state()
+        # the radial profile broadcast to a field is the radial run, bit for bit
+        s.set_c2_field(np.broadcast_to(bg[1][None, :, None, None], (6, nr, n, n)))
+        s.set_state(q0)
+        s.rk4(5, dt)
+        bcast = s.get_state()
+        s.set_c2_field(None)
+        s.set_state(q0)
+        s.rk4(5, dt)
+        radial = s.get_state()
+    assert np.array_equal(got, want)
+    assert np.array_equal(bcast, radial)
+    assert not np.array_equal(got, radial)
+
+
+def _gradient_problem(n=16, nr=24, steps=24, nrec=8, seed=2):
+    cfg, bg, dt = _setup(n, nr, 1, steps)
+    q0, _ = generate_ic(0, n, nr, bg[0], bg[1], 77, [1e-3, 0.05, 0.0])
+    rng = np.random.default_rng(seed)
+    rec = np.stack([rng.integers(0, 6, nrec), rng.integers(4, nr - 4, nrec), rng.integers(0, n, nrec),
+                    rng.integers(0, n, nrec)], axis=1)
+    # "true" model: the radial profile with a 2% Gaussian blob
+    xyz_k = np.arange(nr)[None, :, None, None]
+    blob = np.exp(-((xyz_k - nr / 2) / 4.0) ** 2) * np.ones((6, nr, n, n))
+    c2_0 = np.broadcast_to(bg[1][None, :, None, None], (6, nr, n, n)).copy()
+    c2_true = c2_0 * (1.0 + 0.02 * blob)
+    return cfg, bg, dt, q0, rec, c2_0, c2_true
+
+
+def test_misfit_gradient_matches_finite_differences():
+    cfg, bg, dt, q0, rec, c2_0, c2_true = _gradient_problem()
+    steps = cfg.steps
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_receivers(rec)
+        s.set_c2_field(c2_true)
+        s.set_state(q0)
+        data = s.record(steps, dt)
+
+        def J(c2):
+            s.set_c2_field(c2)
+            s.set_state(q0)
+            d = s.record(steps, dt) - data
+            return float(np.sum(d * d))
+
+        s.set_c2_field(c2_0)
+        J0, grad, seis = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=5)
+        assert J0 > 0
+        # same traces; only the order of the final sum differs (host loop vs numpy)
+        assert abs(J0 - J(c2_0)) <= 1e-13 * J0
+        print(f"[adj] J0={J0:.6e}", flush=True)
+        _, grad_one_segment, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=1000)
+        assert np.allclose(grad, grad_one_segment, rtol=1e-12, atol=1e-12 * np.abs(grad).max())
+
+        rng = np.random.default_rng(4)
+        errs = []
+        for t in range(8):
+            d = rng.standard_normal(c2_0.shape) * c2_0
+            eps = 1e-5
+            fd = (J(c2_0 + eps * d) - J(c2_0 - eps * d)) / (2 * eps)
+            ad = float(np.vdot(grad, d))
+            errs.append(abs(fd - ad) / abs(fd))
+            print(f"[adj] grad dir {t}: fd={fd:.12e} adj={ad:.12e} rel={errs[-1]:.2e}", flush=True)
+    assert max(errs) < 1e-6, errs
