@@ -57,6 +57,64 @@ def _cpu_baseline(cfg, bg, q0, dt, sample_steps):
                       f"after one warm-up step, C restatement of the reference with OpenMP, {el:.1f} s"}
 
 
+def _adjoint_bench(fd_check):
+    """Sensitivity workload: misfit of 16 p' receivers at r = 0.98 over the c5
+    run (6x96x96x192, 2000 steps), data from a model whose c^2 carries a 2%
+    Gaussian blob; gradient dJ/dc^2 by checkpointed discrete adjoint.  Reports
+    the wall time of misfit_gradient over that of the plain forward run."""
+    import numpy as np
+    from solar_amd import CONFIGS, Solver, make_inputs, panel_points
+    cfg = CONFIGS["c5"]
+    bg, q0, dt, _ = make_inputs(cfg)
+    gr, c2 = bg[0], bg[1]
+    n, nr, steps = cfg.n, cfg.nr, cfg.steps
+    kr = int(np.argmin(np.abs(gr - 0.98)))
+    spots = ((n // 4, n // 4), (3 * n // 4, 3 * n // 4), (n // 2, n // 5))
+    rec = np.array([[p, kr, j, i] for p in range(6) for (j, i) in spots][:16])  # 16 receivers
+    xyz = panel_points(n)  # (6, n, n, 3) unit vectors
+    centre = np.array([1.0, 0.3, 0.2]) / np.linalg.norm([1.0, 0.3, 0.2])
+    rc = 0.85
+    pos = gr[None, :, None, None, None] * xyz[:, None, :, :, :]
+    d2 = np.sum((pos - rc * centre) ** 2, axis=-1)
+    blob = np.exp(-d2 / (2 * 0.08 ** 2))
+    c2_true = c2[None, :, None, None] * (1.0 + 0.02 * blob)
+    out = {"workload": "c5: 6x96x96x192, stratified, 2000 RK4 steps, FP64", "steps": steps,
+           "receivers": int(rec.shape[0]), "receiver_radius": float(gr[kr])}
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_receivers(rec)
+        s.set_c2_field(c2_true)
+        s.set_state(q0)
+        data = s.record(steps, dt)
+        s.set_c2_field(None)
+        s.set_state(q0)
+        s.rk4(5, dt)
+        s.set_state(q0)
+        s.sync()
+        t = time.perf_counter()
+        s.rk4(steps, dt)
+        fwd = time.perf_counter() - t
+        t = time.perf_counter()
+        J, grad, seis = s.misfit_gradient(q0, steps, dt, data)
+        tot = time.perf_counter() - t
+        out.update({"fwd_s": fwd, "fwd_adj_s": tot, "ratio": tot / fwd, "segment_steps": s.segment_used,
+                    "J": J, "grad_finite": bool(np.isfinite(grad).all())})
+        if fd_check:
+            # one central difference along a smooth direction (the blob itself)
+            dirn = c2[None, :, None, None] * blob
+            eps = 1e-4
+
+            def J_at(field):
+                s.set_c2_field(field)
+                s.set_state(q0)
+                r = s.record(steps, dt) - data
+                return float(np.sum(r * r))
+            base = np.broadcast_to(c2[None, :, None, None], dirn.shape)
+            fd = (J_at(base + eps * dirn) - J_at(base - eps * dirn)) / (2 * eps)
+            ad = float(np.vdot(grad, dirn))
+            out.update({"fd_dJ": fd, "adj_dJ": ad, "grad_rel_err": abs(fd - ad) / abs(fd)})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +126,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--state-bits", type=int, default=64, choices=(64, 32),
                     help="storage of the state and 2N accumulator (arithmetic is FP64 either way)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint cost measurement")
+    ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference gradient check")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run STEPS steps in FP64 and FP32 storage and report the relative L2 error "
                          "of each field of the FP32 run against the FP64 run")
@@ -135,6 +195,10 @@ def main():
             ["rho", "v_xi", "v_eta", "v_r", "p"], errs))}
         del fin
 
+    adjoint = None
+    if not args.no_adjoint and bits == 64:
+        adjoint = _adjoint_bench(not args.no_adjoint_fd)
+
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(cfg, bg, q0, dt, args.cpu_sample_steps)
@@ -147,6 +211,8 @@ def main():
            "roofline": roofline, "cpu_baseline": cpu}
     if precision:
         out["precision"] = precision
+    if adjoint:
+        out["adjoint"] = adjoint
     if cpu:
         out["gpu_over_cpu"] = rate / cpu["value"]
     print(json.dumps(out), flush=True)

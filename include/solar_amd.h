@@ -156,16 +156,22 @@ int solar_adj_rhs(solar_ctx* ctx, const double* w, double* out);
 int solar_adj_filter(solar_ctx* ctx, const double* o, double* out);
 int solar_adj_step(solar_ctx* ctx, double dt, const double* lam, double* out);
 
-/* Misfit J = sum over steps and receivers of (p'(rec) - data)^2 after nsteps
- * RK4 steps from q0, and its gradient grad(n,n,nr,6) = dJ/dc2 with respect
- * to the c^2 field in use (radial profile or solar_set_c2_field).  The
- * forward pass keeps one state checkpoint per checkpoint_every steps; the
- * reverse pass recomputes each segment, storing div(v) of every stage, and
- * runs the adjoint steps over it.  data[nsteps][nrec]; seis (may be NULL)
- * receives the modelled traces.  Leaves the final forward state of the
- * first segment in the handle's state. */
+/* Misfit J = sum over steps n and receivers r of (p'_n(r) - data[n][r])^2,
+ * p'_n the state after step n+1 of nsteps RK4 steps from q0, and its
+ * gradient grad(n,n,nr,6) = dJ/dc2 with respect to the c^2 field in use
+ * (radial profile or solar_set_c2_field).  Checkpoint/recompute: the steps
+ * are cut into segments of checkpoint_every steps (<= 0: the longest that
+ * fits in free device memory), the last segment ending at nsteps.  The
+ * forward pass keeps the state at the start of every other segment and the
+ * stage divergences div(v) of the last one; the reverse pass recomputes each
+ * earlier segment once from its checkpoint, storing its divergences, and runs
+ * the adjoint steps over it.  data[nsteps][nrec]; seis (may be NULL) receives
+ * the modelled traces; *segment_used (may be NULL) the segment length.
+ * Overwrites the handle's state.  Device memory beyond the handle's:
+ * segment * 40 B per point for the divergences plus one state per earlier
+ * segment. */
 int solar_misfit_gradient(solar_ctx* ctx, const double* q0, int nsteps, double dt, const double* data,
-                          int checkpoint_every, double* J, double* grad, double* seis);
+                          int checkpoint_every, double* J, double* grad, double* seis, int* segment_used);
 
 /* Description of the last error on this thread ("" if none). */
 const char* solar_last_error(void);

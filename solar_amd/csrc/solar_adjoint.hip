@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "solar_adjoint.hpp"
+#include "solar_device.hpp"
 
 namespace solar {
 
@@ -98,8 +99,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
                                                        double* __restrict__ gc2, const double* __restrict__ c2f,
                                                        double* __restrict__ gxa_p, double* __restrict__ gya_p,
                                                        double* __restrict__ gxa_v, double* __restrict__ gya_v,
-                                                       double dt, double alpha, double beta, int kchunk, int nkch,
-                                                       int mode) {
+                                                       double dt, double alpha, double beta, double wscale,
+                                                       int kchunk, int nkch, int mode) {
   __shared__ double sxv[TJ][TI + 6], sxp[TJ][TI + 6];
   __shared__ double syv[TJ + 6][TI], syp[TJ + 6][TI];
 
@@ -121,8 +122,10 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t ij = (size_t)j * n + i;
-  const double* w0 = W + (size_t)(P * 5) * nf;
-  auto wat = [&](int f, int k, size_t cell) { return w0[(size_t)f * nf + (size_t)k * plane + cell]; };
+  // W^(s), or wscale * Ybar when the stage starts from Ybar alone (W == nullptr)
+  const double* w0 = (W ? W : Yin) + (size_t)(P * 5) * nf;
+  const double ws = W ? 1.0 : wscale;
+  auto wat = [&](int f, int k, size_t cell) { return ws * w0[(size_t)f * nf + (size_t)k * plane + cell]; };
 
   // Radial windows of the transposed-DDR inputs over planes k-3..k+3, plus the
   // three closure rows at each shell end (their stencils reach planes 0..6 and
@@ -165,8 +168,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     const RadialRow& row = T.rad[k];
     const size_t kp = (size_t)k * plane;
     // own point
-    const double w1 = w0[kp + ij], w2 = w0[nf + kp + ij], w3 = w0[2 * nf + kp + ij], w4 = w0[3 * nf + kp + ij],
-                 w5 = w0[4 * nf + kp + ij];
+    const double w1 = wat(0, k, ij), w2 = wat(1, k, ij), w3 = wat(2, k, ij), w4 = wat(3, k, ij), w5 = wat(4, k, ij);
     AdjU u;
     adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), w1, w2, w3, w4, w5, u);
     sxv[ty][tx + 3] = u.uxv;
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
 // Transpose of k_gather: one block per (receiver panel P, edge e, plane k).
 // Adds the ghost adjoints to the sender's rows of Y (and beta times them to
 // Wn when given).  Scalars: field sfield (+blockIdx.y); vectors: fields 1, 2.
-template <bool VEC>
+template <bool VEC, bool RADF>
 __global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* __restrict__ gxs,
                                                      const double* __restrict__ gys, const double* __restrict__ gxv,
                                                      const double* __restrict__ gyv, double* Y, double* Wn,
@@ -378,7 +380,19 @@ __global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* 
       o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
     else
       o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
-    const double gs = (e < 2) ? gxs[o] : gys[o];
+    double gs;
+    if (RADF) {
+      // FILTER_R^T of the per-plane ghost adjoints: planes kk in [5, nr-6]
+      // within 5 of k, weight filt_r[k - kk + 5] (filter.f90:682-693)
+      const double* gsrc = (e < 2) ? gxs : gys;
+      const size_t kst = (e < 2) ? (size_t)n * 3 : (size_t)3 * n;
+      const size_t o0 = o - (size_t)k * kst;
+      gs = 0.0;
+      for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
+        gs += T.filtr[k - kk + 5] * gsrc[o0 + (size_t)kk * kst];
+    } else {
+      gs = (e < 2) ? gxs[o] : gys[o];
+    }
     double th = 0.0, ph = 0.0;
     const int mp = rt.reverse ? n - 1 - m : m;
     if (VEC) {
@@ -411,96 +425,146 @@ __global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* 
   }
 }
 
-// FILTER_R^T per point: out(k') = sum over k in [5, nr-6], |k-k'| <= 5 of
-// filt_r[k'-k+5] in(k)  (filter.f90:682-693; the end planes carry no value).
-__global__ void k_adj_filter_r(DevTables T, const double* __restrict__ in, double* __restrict__ out) {
-  const int n = T.n, nr = T.nr;
-  const size_t plane = (size_t)n * n, nf = plane * nr, total = nf * 30;
-  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
-    const size_t fo = x / nf * nf, r = x % nf;
-    const int k = (int)(r / plane);
-    const size_t c = r % plane;
-    double s = 0.0;
-    for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
-      s += T.filtr[k - kk + 5] * in[fo + (size_t)kk * plane + c];
-    out[x] = s;
-  }
-}
-
-// FILTER_ETA^T then FILTER_XI^T on one plane of one field (zero extension),
-// with the ghost adjoints of both passes for edge tiles (scaled by scale).
+// FILTER_EXP^T = (FILTER_R FILTER_ETA FILTER_XI)^T.  FILTER_R acts on the
+// radial index alone and the two angular filters (with their ghost exchange)
+// on each plane alone, so the transpose is taken as FILTER_R^T applied to
+// FILTER_XI^T FILTER_ETA^T of each plane (equal in exact arithmetic).
+// Per plane kk in [5, nr-6] (FILTER_R reads no other plane's output,
+// filter.f90:682-693): the (TJ+6)x(TI+6) tile of bbar (zero beyond the panel:
+// the transposed stencils have no ghosts) goes to LDS, FILTER_ETA^T runs on
+// every column, FILTER_XI^T on the in-panel rows; the result enters an
+// 11-plane register window from which FILTER_R^T forms the output plane
+// kk - 5.  Edge tiles write the per-plane ghost adjoints of both angular
+// passes (the eta ghosts from bbar itself: FILTER_ETA reads unfiltered
+// ghosts, filter.f90:43); planes outside [5, nr-6] get zeros.
 template <int TI, int TJ>
-__global__ __launch_bounds__(TI* TJ) void k_adj_filter_xe(DevTables T, const double* __restrict__ bin,
-                                                           double* __restrict__ out, double* __restrict__ fgxa,
-                                                           double* __restrict__ fgya) {
+__global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double* __restrict__ bin,
+                                                        double* __restrict__ out, double* __restrict__ fgxa,
+                                                        double* __restrict__ fgya, int kchunk, int nkch) {
+  constexpr int NT = TI * TJ;
+  constexpr int NCELL = (TJ + 6) * (TI + 6);
+  constexpr int NC = (NCELL + NT - 1) / NT;
   __shared__ double sb[TJ + 6][TI + 6];
   __shared__ double sa[TJ][TI + 6];
   const int n = T.n, nr = T.nr;
   const int nti = n / TI, ntj = n / TJ;
-  int b = blockIdx.x;
+  const int nb = nti * ntj * 30 * nkch;
+  int b = xcd_remap(blockIdx.x, nb);
   const int bi = b % nti;
   b /= nti;
   const int bj = b % ntj;
   b /= ntj;
   const int PF = b % 30;
-  const int k = b / 30;
+  const int ch = b / 30;
   const int P = PF / 5, F = PF % 5;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
+  const int k0 = ch * kchunk;
+  const int k1 = min(k0 + kchunk, nr);
+  if (k0 >= nr) return;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const double* g = bin + (size_t)PF * nf + (size_t)k * plane;
-  for (int e = ty * TI + tx; e < (TJ + 6) * (TI + 6); e += TI * TJ) {
-    const int rr = e / (TI + 6), cc = e % (TI + 6);
-    const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
-    sb[rr][cc] = (gj >= 0 && gj < n && gi >= 0 && gi < n) ? g[(size_t)gj * n + gi] : 0.0;
-  }
-  __syncthreads();
-  // abar(row ty, column c) = sum_q fe[6-q] bbar(row ty-3+q)
-  for (int cc = tx; cc < TI + 6; cc += TI) {
-    double a = 0.0;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) a += T.filt[6 - q] * sb[ty + q][cc];
-    sa[ty][cc] = a;
-  }
-  __syncthreads();
-  double qb = 0.0;
-#pragma unroll
-  for (int q = 0; q < 7; ++q) qb += T.filt[6 - q] * sa[ty][tx + q];
-  out[(size_t)PF * nf + (size_t)k * plane + (size_t)j * n + i] = qb;
-
+  const double* g = bin + (size_t)PF * nf;
+  double* o = out + (size_t)PF * nf;
+  const size_t ij = (size_t)j * n + i;
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
   const size_t fst = (size_t)12 * gx_side;
-  if (bj == 0 && ty < 3) {  // eta-pass ghosts below the S edge
-    const int l = ty + 1;
-    double s = 0.0;
-    for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * sb[3 + jj][tx + 3];
-    fgya[F * fst + (size_t)(P * 2) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i] = s;
-  }
-  if (bj == ntj - 1 && ty >= TJ - 3) {
-    const int l = ty - (TJ - 3) + 1;
-    double s = 0.0;
-    for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * sb[3 + jj - j0][tx + 3];
-    fgya[F * fst + (size_t)(P * 2 + 1) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i] = s;
-  }
-  if (bi == 0 && tx < 3) {  // xi-pass ghosts west of the W edge
-    const int l = tx + 1;
-    double s = 0.0;
-    for (int ii = 0; ii <= 3 - l; ++ii) s += T.filt[3 - l - ii] * sa[ty][3 + ii];
-    fgxa[F * fst + (size_t)(P * 2) * gx_side + ((size_t)k * n + j) * 3 + (l - 1)] = s;
-  }
-  if (bi == nti - 1 && tx >= TI - 3) {
-    const int l = tx - (TI - 3) + 1;
-    double s = 0.0;
-    for (int ii = n - 4 + l; ii <= n - 1; ++ii) s += T.filt[n + 2 + l - ii] * sa[ty][3 + ii - i0];
-    fgxa[F * fst + (size_t)(P * 2 + 1) * gx_side + ((size_t)k * n + j) * 3 + (l - 1)] = s;
-  }
-}
+  const bool gW = bi == 0 && tx < 3, gE = bi == nti - 1 && tx >= TI - 3;
+  const bool gS = bj == 0 && ty < 3, gN = bj == ntj - 1 && ty >= TJ - 3;
+  const int lx = gW ? tx + 1 : tx - (TI - 3) + 1, ly = gS ? ty + 1 : ty - (TJ - 3) + 1;
+  double* gxo = fgxa + F * fst + (size_t)(P * 2 + (gW ? 0 : 1)) * gx_side + (size_t)j * 3 + (lx - 1);
+  double* gyo = fgya + F * fst + (size_t)(P * 2 + (gS ? 0 : 1)) * gy_side + (size_t)(ly - 1) * n + i;
 
-// y = a*x (whole 5-field state).
-__global__ void k_scale(const double* __restrict__ x, double* __restrict__ y, double a, size_t total) {
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
-    y[t] = a * x[t];
+  // ghost adjoints of the planes of this chunk that carry none
+  for (int k = k0; k < k1; ++k)
+    if (k < kNstr || k >= nr - kNstr) {
+      if (gW || gE) gxo[(size_t)k * n * 3] = 0.0;
+      if (gS || gN) gyo[(size_t)k * 3 * n] = 0.0;
+    }
+
+  // cells of the tile this thread loads
+  const int tid = ty * TI + tx;
+  long long coff[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int e = tid + c * NT;
+    coff[c] = -1;
+    if (e < NCELL) {
+      const int rr = e / (TI + 6), cc = e % (TI + 6);
+      const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
+      if (gj >= 0 && gj < n && gi >= 0 && gi < n) coff[c] = (long long)gj * n + gi;
+    }
+  }
+  auto load_plane = [&](int kk, double (&dst)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) dst[c] = coff[c] >= 0 ? g[(size_t)kk * plane + coff[c]] : 0.0;
+  };
+
+  double win[11];
+#pragma unroll
+  for (int q = 0; q < 11; ++q) win[q] = 0.0;
+  // planes kk carrying FILTER_R output, kk in [max(k0-5,5), min(k1+4,nr-6)];
+  // the march continues to k1+4 so that output plane k1-1 is formed.
+  const int kfirst = max(k0 - kNstr, kNstr), klast = min(k1 + kNstr - 1, nr - 1 - kNstr);
+  const int kstop = k1 + kNstr - 1;
+  double nx[NC];
+  if (kfirst <= klast) load_plane(kfirst, nx);
+  for (int kk = kfirst; kk <= kstop; ++kk) {
+    double v = 0.0;
+    if (kk <= klast) {  // block-uniform
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const int e = tid + c * NT;
+        if (e < NCELL) (&sb[0][0])[e] = nx[c];
+      }
+      if (kk + 1 <= klast) load_plane(kk + 1, nx);
+      __syncthreads();
+      // FILTER_ETA^T: abar(row, c) = sum_q f[6-q] bbar(row-3+q, c), in-panel rows
+      for (int cc = tx; cc < TI + 6; cc += TI) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) a += T.filt[6 - q] * sb[ty + q][cc];
+        sa[ty][cc] = a;
+      }
+      __syncthreads();
+      // FILTER_XI^T
+#pragma unroll
+      for (int q = 0; q < 7; ++q) v += T.filt[6 - q] * sa[ty][tx + q];
+      if (kk >= k0 && kk < k1) {
+        if (gS || gN) {  // eta ghost l sits at row -l (S) or n-1+l (N)
+          double s = 0.0;
+          if (gS) {
+            for (int jj = 0; jj <= 3 - ly; ++jj) s += T.filt[3 - ly - jj] * sb[3 + jj][tx + 3];
+          } else {
+            for (int jj = n - 4 + ly; jj <= n - 1; ++jj) s += T.filt[n + 2 + ly - jj] * sb[3 + jj - j0][tx + 3];
+          }
+          gyo[(size_t)kk * 3 * n] = s;
+        }
+        if (gW || gE) {
+          double s = 0.0;
+          if (gW) {
+            for (int ii = 0; ii <= 3 - lx; ++ii) s += T.filt[3 - lx - ii] * sa[ty][3 + ii];
+          } else {
+            for (int ii = n - 4 + lx; ii <= n - 1; ++ii) s += T.filt[n + 2 + lx - ii] * sa[ty][3 + ii - i0];
+          }
+          gxo[(size_t)kk * n * 3] = s;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) win[q] = win[q + 1];
+    win[10] = v;
+    // win[q] = A(kk-10+q); output plane kk-5 takes filt_r[10-q] * win[q]
+    const int kout = kk - kNstr;
+    if (kout >= k0 && kout < k1) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < 11; ++q) s += T.filtr[10 - q] * win[q];
+      o[(size_t)kout * plane + ij] = s;
+    }
+  }
+  // kfirst - 5 <= k0 and kstop - 5 = k1 - 1: every plane of [k0, k1) was written
 }
 
 // Receiver sampling and adjoint sources.
@@ -528,8 +592,8 @@ static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, 
   if (nkch < 1) nkch = 1;
   const int kchunk = (T.nr + nkch - 1) / nkch;
   hipLaunchKernelGGL((k_adj_stage<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout, a.Wout,
-                     a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, kchunk, nkch,
-                     a.mode);
+                     a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, a.wscale,
+                     kchunk, nkch, a.mode);
   return hipGetLastError();
 }
 
@@ -540,34 +604,39 @@ hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream
 
 hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
                               const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
-                              hipStream_t st) {
+                              hipStream_t st, bool radial_filter) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
   dim3 grid((unsigned)(24 * T.nr), (unsigned)nfields);
   if (vec)
-    hipLaunchKernelGGL(k_adj_scatter<true>, grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield,
-                       field_stride);
+    hipLaunchKernelGGL((k_adj_scatter<true, false>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                       sfield, field_stride);
+  else if (radial_filter)
+    hipLaunchKernelGGL((k_adj_scatter<false, true>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                       sfield, field_stride);
   else
-    hipLaunchKernelGGL(k_adj_scatter<false>, grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield,
-                       field_stride);
+    hipLaunchKernelGGL((k_adj_scatter<false, false>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                       sfield, field_stride);
   return hipGetLastError();
 }
 
-hipError_t launch_adj_filter(const DevTables& T, const double* in, double* tmp, double* out, double* fgxa,
-                             double* fgya, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_filter_r, dim3(4096), dim3(256), 0, st, T, in, tmp);
-  if (adj_ti(T.n) == 32) {
-    const int tiles = (T.n / 32) * (T.n / 8) * 30 * T.nr;
-    hipLaunchKernelGGL((k_adj_filter_xe<32, 8>), dim3(tiles), dim3(32, 8), 0, st, T, tmp, out, fgxa, fgya);
-  } else {
-    const int tiles = (T.n / 16) * (T.n / 16) * 30 * T.nr;
-    hipLaunchKernelGGL((k_adj_filter_xe<16, 16>), dim3(tiles), dim3(16, 16), 0, st, T, tmp, out, fgxa, fgya);
-  }
+template <int TI, int TJ>
+static hipError_t launch_adj_filter_t(const DevTables& T, const double* in, double* out, double* fgxa, double* fgya,
+                                      hipStream_t st) {
+  const int tiles = (T.n / TI) * (T.n / TJ) * 30;
+  // chunks of >= 48 planes: the march re-reads 10 planes per chunk
+  int nkch = (2048 + tiles - 1) / tiles;
+  const int maxk = T.nr / 48 > 0 ? T.nr / 48 : 1;
+  if (nkch > maxk) nkch = maxk;
+  const int kchunk = (T.nr + nkch - 1) / nkch;
+  hipLaunchKernelGGL((k_adj_filter<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, in, out, fgxa, fgya, kchunk,
+                     nkch);
   return hipGetLastError();
 }
 
-hipError_t launch_scale(const double* x, double* y, double a, size_t total, hipStream_t st) {
-  hipLaunchKernelGGL(k_scale, dim3(4096), dim3(256), 0, st, x, y, a, total);
-  return hipGetLastError();
+hipError_t launch_adj_filter(const DevTables& T, const double* in, double* out, double* fgxa, double* fgya,
+                             hipStream_t st) {
+  if (T.n % 32 == 0) return launch_adj_filter_t<32, 16>(T, in, out, fgxa, fgya, st);
+  return launch_adj_filter_t<16, 16>(T, in, out, fgxa, fgya, st);
 }
 
 hipError_t launch_sample(const double* q, const long long* idx, int nrec, double* out, hipStream_t st) {

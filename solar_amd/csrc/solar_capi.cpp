@@ -68,7 +68,7 @@ struct solar_ctx {
   double* c2f = nullptr;  // optional 3-D c^2 (n,n,nr,6); points at c2f_buf when set
   double* c2f_buf = nullptr;
   double* c2f_graph = nullptr;
-  double *wb[2] = {nullptr, nullptr}, *yb = nullptr, *tmp = nullptr, *lam = nullptr, *gc2 = nullptr;
+  double *wb[2] = {nullptr, nullptr}, *tmp = nullptr, *lam = nullptr, *gc2 = nullptr;
   long long* rec = nullptr;  // receiver indices into the state (field p')
   int nrec = 0, rec_cap = 0;
 };
@@ -515,12 +515,12 @@ int solar_host_tables(int n, int nr, const double* gr, const double* c2, const d
 // ------------------------------------------------------------ sensitivities
 
 static int adj_alloc(solar_ctx* c) {
-  if (c->yb) return SOLAR_OK;
+  if (c->lam) return SOLAR_OK;
   if (c->f32) return fail(SOLAR_ERR_ARG, "adjoint: needs a 64-bit state handle");
   const size_t nq = c->nf * 30;
   int rc;
-  if ((rc = dev_alloc(c, &c->wb[0], nq)) || (rc = dev_alloc(c, &c->wb[1], nq)) || (rc = dev_alloc(c, &c->yb, nq)) ||
-      (rc = dev_alloc(c, &c->tmp, nq)) || (rc = dev_alloc(c, &c->lam, nq)) || (rc = dev_alloc(c, &c->gc2, c->nf * 6)))
+  if ((rc = dev_alloc(c, &c->wb[0], nq)) || (rc = dev_alloc(c, &c->wb[1], nq)) || (rc = dev_alloc(c, &c->tmp, nq)) ||
+      (rc = dev_alloc(c, &c->lam, nq)) || (rc = dev_alloc(c, &c->gc2, c->nf * 6)))
     return rc;
   return SOLAR_OK;
 }
@@ -535,24 +535,22 @@ static int download_f64(solar_ctx* c, double* dst, const double* src, size_t cou
   return SOLAR_OK;
 }
 
-// F^T then the five reverse stages of one step: out = S^T in.  With divp, the
-// c^2 gradient of this step is added to c->gc2 (divp[s] = div(v) of the
-// forward stage s input).
-static int enqueue_adj_step(solar_ctx* c, double dt, const double* in, double* out, double* const* divp) {
+// lam <- S^T lam for one step: FILTER_EXP^T into tmp, then the five reverse
+// stages in place on lam.  With divp, the c^2 gradient of this step is added
+// to c->gc2 (divp[s] = div(v) of the input of forward stage s, 0-based).
+static int enqueue_adj_step(solar_ctx* c, double dt, double* lam, double* const* divp) {
   const solar::DevTables& T = c->dt;
-  const size_t nq = c->nf * 30;
-  // FILTER_EXP^T: out = F^T in  (tmp holds FILTER_R^T in)
-  HIPCHK(solar::launch_adj_filter(T, in, c->tmp, out, c->fgx, c->fgy, c->stream));
-  HIPCHK(solar::launch_adj_scatter(T, false, c->fgx, c->fgy, nullptr, nullptr, out, nullptr, 0.0, 0, 5, c->stream));
-  // W^(5) = beta_5 Ybar^(5)
-  HIPCHK(solar::launch_scale(out, c->wb[0], solar::kBetaRk4[4], nq, c->stream));
-  int cur = 0;
+  HIPCHK(solar::launch_adj_filter(T, lam, c->tmp, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_scatter(T, false, c->fgx, c->fgy, nullptr, nullptr, c->tmp, nullptr, 0.0, 0, 5, c->stream,
+                                   true));
   for (int s = 5; s >= 1; --s) {
     solar::AdjStageArgs a;
-    a.W = c->wb[cur];
-    a.Yin = out;
-    a.Yout = out;
-    a.Wout = s > 1 ? c->wb[1 - cur] : nullptr;
+    // stage 5 starts from Ybar^(5) = F^T lam alone: W^(5) = beta_5 Ybar^(5)
+    a.W = s == 5 ? nullptr : c->wb[(4 - s) % 2];
+    a.wscale = solar::kBetaRk4[4];
+    a.Yin = s == 5 ? c->tmp : lam;
+    a.Yout = lam;
+    a.Wout = s > 1 ? c->wb[(5 - s) % 2] : nullptr;
     a.dt = dt;
     a.alpha = solar::kAlphaRk4[s - 1];
     a.beta = s > 1 ? solar::kBetaRk4[s - 2] : 0.0;
@@ -565,8 +563,7 @@ static int enqueue_adj_step(solar_ctx* c, double dt, const double* in, double* o
     a.gxa_v = c->gxv;
     a.gya_v = c->gyv;
     HIPCHK(solar::launch_adj_stage(T, a, c->stream));
-    HIPCHK(solar::launch_adj_scatter(T, true, c->gxp, c->gyp, c->gxv, c->gyv, out, a.Wout, a.beta, 4, 1, c->stream));
-    cur = 1 - cur;
+    HIPCHK(solar::launch_adj_scatter(T, true, c->gxp, c->gyp, c->gxv, c->gyv, lam, a.Wout, a.beta, 4, 1, c->stream));
   }
   return SOLAR_OK;
 }
@@ -643,7 +640,7 @@ int solar_adj_rhs(solar_ctx* c, const double* w, double* out) {
   if ((rc = upload_f64(c, c->wb[0], w, nq))) return rc;
   solar::AdjStageArgs a;
   a.W = c->wb[0];
-  a.Yout = c->yb;
+  a.Yout = c->tmp;
   a.c2f = c->c2f;
   a.mode = solar::kAdjRhsOnly;
   a.gxa_p = c->gxp;
@@ -651,8 +648,8 @@ int solar_adj_rhs(solar_ctx* c, const double* w, double* out) {
   a.gxa_v = c->gxv;
   a.gya_v = c->gyv;
   HIPCHK(solar::launch_adj_stage(c->dt, a, c->stream));
-  HIPCHK(solar::launch_adj_scatter(c->dt, true, c->gxp, c->gyp, c->gxv, c->gyv, c->yb, nullptr, 0.0, 4, 1, c->stream));
-  return download_f64(c, out, c->yb, nq);
+  HIPCHK(solar::launch_adj_scatter(c->dt, true, c->gxp, c->gyp, c->gxv, c->gyv, c->tmp, nullptr, 0.0, 4, 1, c->stream));
+  return download_f64(c, out, c->tmp, nq);
 }
 
 int solar_adj_filter(solar_ctx* c, const double* o, double* out) {
@@ -661,9 +658,10 @@ int solar_adj_filter(solar_ctx* c, const double* o, double* out) {
   if (rc) return rc;
   const size_t nq = c->nf * 30;
   if ((rc = upload_f64(c, c->lam, o, nq))) return rc;
-  HIPCHK(solar::launch_adj_filter(c->dt, c->lam, c->tmp, c->yb, c->fgx, c->fgy, c->stream));
-  HIPCHK(solar::launch_adj_scatter(c->dt, false, c->fgx, c->fgy, nullptr, nullptr, c->yb, nullptr, 0.0, 0, 5, c->stream));
-  return download_f64(c, out, c->yb, nq);
+  HIPCHK(solar::launch_adj_filter(c->dt, c->lam, c->tmp, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_scatter(c->dt, false, c->fgx, c->fgy, nullptr, nullptr, c->tmp, nullptr, 0.0, 0, 5,
+                                   c->stream, true));
+  return download_f64(c, out, c->tmp, nq);
 }
 
 int solar_adj_step(solar_ctx* c, double dt, const double* lam, double* out) {
@@ -672,91 +670,108 @@ int solar_adj_step(solar_ctx* c, double dt, const double* lam, double* out) {
   if (rc) return rc;
   const size_t nq = c->nf * 30;
   if ((rc = upload_f64(c, c->lam, lam, nq))) return rc;
-  if ((rc = enqueue_adj_step(c, dt, c->lam, c->yb, nullptr))) return rc;
-  return download_f64(c, out, c->yb, nq);
+  if ((rc = enqueue_adj_step(c, dt, c->lam, nullptr))) return rc;
+  return download_f64(c, out, c->lam, nq);
 }
 
 int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt, const double* data,
-                          int checkpoint_every, double* J, double* grad, double* seis_out) {
-  if (!c || !q0 || !data || !J || !grad || nsteps < 1 || checkpoint_every < 1)
+                          int checkpoint_every, double* J, double* grad, double* seis_out, int* segment_used) {
+  if (!c || !q0 || !data || !J || !grad || nsteps < 1)
     return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: invalid argument");
   if (c->nrec < 1) return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: no receivers set");
   int rc = adj_alloc(c);
   if (rc) return rc;
   const size_t nq = c->nf * 30, nf6 = c->nf * 6;
-  const int seg = checkpoint_every < nsteps ? checkpoint_every : nsteps;
-  const int nck = (nsteps + seg - 1) / seg;
-  const size_t need = (static_cast<size_t>(nck) * nq + static_cast<size_t>(seg) * 5 * nf6 +
-                       2 * static_cast<size_t>(nsteps) * c->nrec) * sizeof(double);
+  const size_t step_div = 5 * nf6 * sizeof(double), ck_bytes = nq * sizeof(double);
+  const size_t trace_bytes = 2 * static_cast<size_t>(nsteps) * c->nrec * sizeof(double);
+  // Segment length L: checkpoint_every, or (<= 0) the longest that fits.  The
+  // last L steps keep their stage divergences from the forward pass; every
+  // earlier segment is recomputed once from its checkpoint.
+  auto need = [&](int L) {
+    const size_t nseg = (static_cast<size_t>(nsteps) + L - 1) / L;
+    return static_cast<size_t>(L) * step_div + (nseg - 1) * ck_bytes + trace_bytes;
+  };
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
-  if (need + (size_t(1) << 30) > freeb)
-    return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: needs " + std::to_string(need >> 20) + " MiB of checkpoints, " +
-                                   std::to_string(freeb >> 20) + " MiB free; raise checkpoint_every");
+  const size_t margin = size_t(2) << 30;
+  const size_t avail = freeb > margin ? freeb - margin : 0;
+  int L = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
+  if (checkpoint_every <= 0)
+    while (L > 1 && need(L) > avail) --L;
+  if (need(L) > avail)
+    return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: needs " + std::to_string(need(L) >> 20) + " MiB, " +
+                                   std::to_string(avail >> 20) + " MiB free" +
+                                   (checkpoint_every > 0 ? "; lower checkpoint_every" : ""));
+  const int nseg = (nsteps + L - 1) / L;
+  std::vector<int> n0(nseg), n1(nseg);  // segment g covers steps [n0, n1); the last is full
+  for (int g = 0; g < nseg; ++g) {
+    n1[g] = nsteps - (nseg - 1 - g) * L;
+    n0[g] = std::max(0, n1[g] - L);
+  }
   double *ck = nullptr, *divs = nullptr, *seis = nullptr, *dat = nullptr;
   std::vector<void*> mine;
-  auto grab = [&](double** p, size_t count) -> int {
+  auto grab = [&](double** p, size_t bytes) -> int {
     void* v = nullptr;
-    hipError_t e = hipMalloc(&v, count * sizeof(double));
+    hipError_t e = hipMalloc(&v, bytes ? bytes : 8);
     if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     mine.push_back(v);
     *p = static_cast<double*>(v);
     return SOLAR_OK;
   };
-  auto cleanup = [&]() {
+  if ((rc = grab(&ck, (nseg - 1) * ck_bytes)) || (rc = grab(&divs, static_cast<size_t>(L) * step_div)) ||
+      (rc = grab(&seis, trace_bytes / 2)) || (rc = grab(&dat, trace_bytes / 2))) {
     for (void* v : mine) (void)hipFree(v);
-  };
-  if ((rc = grab(&ck, static_cast<size_t>(nck) * nq)) || (rc = grab(&divs, static_cast<size_t>(seg) * 5 * nf6)) ||
-      (rc = grab(&seis, static_cast<size_t>(nsteps) * c->nrec)) ||
-      (rc = grab(&dat, static_cast<size_t>(nsteps) * c->nrec))) {
-    cleanup();
     return rc;
   }
+  std::vector<double*> dp(static_cast<size_t>(L) * 5);
+  for (size_t x = 0; x < dp.size(); ++x) dp[x] = divs + x * nf6;
+  const size_t nrec = static_cast<size_t>(c->nrec);
   auto run = [&]() -> int {
     int r;
-    if ((r = upload_f64(c, dat, data, static_cast<size_t>(nsteps) * c->nrec))) return r;
+    if ((r = upload_f64(c, dat, data, static_cast<size_t>(nsteps) * nrec))) return r;
     if ((r = upload_state(c, c->q[0], q0, nq))) return r;
-    // forward pass: checkpoints and receiver traces
-    for (int n = 0; n < nsteps; ++n) {
-      if (n % seg == 0)
-        HIPCHK(hipMemcpyAsync(ck + static_cast<size_t>(n / seg) * nq, c->q[0], nq * sizeof(double),
-                              hipMemcpyDeviceToDevice, c->stream));
-      if ((r = solar_rk4_async(c, 1, dt))) return r;
-      HIPCHK(solar::launch_sample(static_cast<const double*>(c->q[0]), c->rec, c->nrec,
-                                  seis + static_cast<size_t>(n) * c->nrec, c->stream));
+    // forward pass: checkpoints, traces, and the stage divergences of the last segment
+    const int nlast = n0[nseg - 1];
+    for (int n = 0, g = 0; n < nsteps; ++n) {
+      if (g < nseg - 1 && n == n0[g]) {
+        HIPCHK(hipMemcpyAsync(ck + static_cast<size_t>(g) * nq, c->q[0], ck_bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+        ++g;
+      }
+      if (n >= nlast) {
+        if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(n - nlast) * 5]))) return r;
+      } else if ((r = solar_rk4_async(c, 1, dt))) {
+        return r;
+      }
+      HIPCHK(solar::launch_sample(static_cast<const double*>(c->q[0]), c->rec, c->nrec, seis + n * nrec, c->stream));
     }
-    std::vector<double> sh(static_cast<size_t>(nsteps) * c->nrec);
+    std::vector<double> sh(static_cast<size_t>(nsteps) * nrec);
     if ((r = download_f64(c, sh.data(), seis, sh.size()))) return r;
     double jj = 0.0;
     for (size_t x = 0; x < sh.size(); ++x) jj += (sh[x] - data[x]) * (sh[x] - data[x]);
     *J = jj;
     if (seis_out) std::memcpy(seis_out, sh.data(), sh.size() * sizeof(double));
-    // reverse pass, one checkpoint segment at a time
+    // reverse pass
     HIPCHK(hipMemsetAsync(c->lam, 0, nq * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->gc2, 0, nf6 * sizeof(double), c->stream));
-    double* lam = c->lam;
-    double* other = c->yb;
-    std::vector<double*> dp(static_cast<size_t>(seg) * 5);
-    for (size_t x = 0; x < dp.size(); ++x) dp[x] = divs + x * nf6;
-    for (int g = nck - 1; g >= 0; --g) {
-      const int n0 = g * seg, n1 = std::min(nsteps, n0 + seg);
-      HIPCHK(hipMemcpyAsync(c->q[0], ck + static_cast<size_t>(g) * nq, nq * sizeof(double), hipMemcpyDeviceToDevice,
-                            c->stream));
-      for (int m = n0; m < n1; ++m)
-        if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - n0) * 5]))) return r;
-      for (int n = n1 - 1; n >= n0; --n) {
-        HIPCHK(solar::launch_inject(lam, c->rec, c->nrec, seis + static_cast<size_t>(n) * c->nrec,
-                                    dat + static_cast<size_t>(n) * c->nrec, c->stream));
-        // lambda_n = S_n^T lambda_{n+1}: reads `lam`, writes `other`, then they swap
-        if ((r = enqueue_adj_step(c, dt, lam, other, &dp[static_cast<size_t>(n - n0) * 5]))) return r;
-        std::swap(lam, other);
+    for (int g = nseg - 1; g >= 0; --g) {
+      if (g < nseg - 1) {  // recompute the segment from its checkpoint, keeping the divergences
+        HIPCHK(hipMemcpyAsync(c->q[0], ck + static_cast<size_t>(g) * nq, ck_bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+        for (int m = n0[g]; m < n1[g]; ++m)
+          if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - n0[g]) * 5]))) return r;
+      }
+      for (int n = n1[g] - 1; n >= n0[g]; --n) {
+        HIPCHK(solar::launch_inject(c->lam, c->rec, c->nrec, seis + n * nrec, dat + n * nrec, c->stream));
+        if ((r = enqueue_adj_step(c, dt, c->lam, &dp[static_cast<size_t>(n - n0[g]) * 5]))) return r;
       }
     }
     return download_f64(c, grad, c->gc2, nf6);
   };
   rc = run();
   (void)hipStreamSynchronize(c->stream);
-  cleanup();
+  for (void* v : mine) (void)hipFree(v);
+  if (segment_used) *segment_used = L;
   return rc;
 }
 }  // extern "C"

@@ -22,6 +22,7 @@
 //               register window.
 #include <hip/hip_runtime.h>
 
+#include "solar_device.hpp"
 #include "solar_kernels.hpp"
 
 namespace solar {
@@ -135,14 +136,6 @@ __global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict
         gyv[o] = g;
     }
   }
-}
-
-// Block index remap so that consecutive logical blocks (neighbouring tiles,
-// whose halos overlap) run on the same XCD and share its L2.  Hardware
-// dispatches block b to XCD b % 8.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  if (nb % 8 != 0) return b;
-  return (b % 8) * (nb / 8) + b / 8;
 }
 
 // Values one thread loads for one radial plane of the stage kernel: its own

@@ -48,7 +48,7 @@ EXPORTS = [
     ("solar_adj_filter", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("solar_adj_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _D, _D]),
     ("solar_misfit_gradient", ctypes.c_int, [ctypes.c_void_p, _D, ctypes.c_int, ctypes.c_double, _D, ctypes.c_int,
-                                             _D, _D, _D]),
+                                             _D, _D, _D, ctypes.POINTER(ctypes.c_int)]),
     ("solar_last_error", ctypes.c_char_p, []),
     ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                          _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -293,16 +293,22 @@ class Solver:
         _check(self.lib.solar_adj_step(self.h, float(dt), _p(self._arr(lam)), _p(out)), "solar_adj_step")
         return out
 
-    def misfit_gradient(self, q0, nsteps, dt, data, checkpoint_every=50):
-        """(J, dJ/dc2 of shape (6, nr, n, n), modelled traces (nsteps, nrec))."""
+    def misfit_gradient(self, q0, nsteps, dt, data, checkpoint_every=0):
+        """(J, dJ/dc2 of shape (6, nr, n, n), modelled traces (nsteps, nrec)).
+
+        checkpoint_every <= 0 picks the longest segment that fits in device
+        memory; self.segment_used holds the length used."""
         data = _f64(data)
         nrec = getattr(self, "nrec", 0)
         if data.size != int(nsteps) * nrec:
             raise SolarError(f"data of {data.size} values, expected ({nsteps}, {nrec})")
         J = ctypes.c_double()
+        seg = ctypes.c_int()
         grad = np.zeros((6, self.nr, self.n, self.n))
         seis = np.zeros((int(nsteps), nrec))
         _check(self.lib.solar_misfit_gradient(self.h, _p(self._arr(q0)), int(nsteps), float(dt), _p(data),
-                                              int(checkpoint_every), ctypes.cast(ctypes.byref(J), _D), _p(grad), _p(seis)),
+                                              int(checkpoint_every), ctypes.cast(ctypes.byref(J), _D), _p(grad), _p(seis),
+                                              ctypes.byref(seg)),
                "solar_misfit_gradient")
+        self.segment_used = seg.value
         return J.value, grad, seis

@@ -162,3 +162,35 @@ def test_misfit_gradient_matches_finite_differences():
             errs.append(abs(fd - ad) / abs(fd))
             print(f"[adj] grad dir {t}: fd={fd:.12e} adj={ad:.12e} rel={errs[-1]:.2e}", flush=True)
     assert max(errs) < 1e-6, errs
+
+
+def test_autograd_harness():
+    import torch
+
+    from solar_amd.autograd import MisfitC2, RK4Step
+    cfg, bg, dt, q0, rec, c2_0, c2_true = _gradient_problem(steps=10)
+    n, nr = cfg.n, cfg.nr
+    rng = np.random.default_rng(12)
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        # two chained steps: d<S S x, lam>/dx = S^T S^T lam
+        x = torch.tensor(rng.standard_normal((6, 5, nr, n, n)), requires_grad=True)
+        lam = torch.tensor(rng.standard_normal((6, 5, nr, n, n)))
+        y = RK4Step.apply(RK4Step.apply(x, s, dt), s, dt)
+        (y * lam).sum().backward()
+        want = s.adj_step(s.adj_step(lam.numpy(), dt), dt)
+        # the panel scatter adds with atomics, so the order of its sums may differ between calls
+        assert np.allclose(x.grad.numpy(), want, rtol=0, atol=1e-13 * np.abs(want).max())
+        lhs = float((y.detach() * lam).sum())
+        rhs = float((x.detach() * x.grad).sum())
+        assert abs(lhs - rhs) <= 1e-13 * float(y.norm() * lam.norm())
+
+        s.set_receivers(rec)
+        s.set_c2_field(c2_true)
+        s.set_state(q0)
+        data = s.record(cfg.steps, dt)
+        c2 = torch.tensor(c2_0, requires_grad=True)
+        J = MisfitC2.apply(c2, s, torch.tensor(q0), cfg.steps, dt, data, 4)
+        (3.0 * J).backward()
+        J2, grad, _ = s.misfit_gradient(q0, cfg.steps, dt, data, checkpoint_every=4)
+    assert abs(float(J) - J2) <= 1e-14 * J2
+    assert np.allclose(c2.grad.numpy(), 3.0 * grad, rtol=1e-12, atol=1e-12 * np.abs(grad).max())
