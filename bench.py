@@ -115,6 +115,50 @@ def _adjoint_bench(fd_check):
     return out
 
 
+def _ensemble_bench(steps, reps):
+    """Batched workload: 48 independent Gaussian-pulse simulations, uniform
+    background, angular n and radial nr drawn from {32,48,64,96} x {64,128,192},
+    pulse widths in [0.02, 0.08] and source positions from the seed; FP64;
+    every step one hipGraph replay with one launch per kernel class and tile
+    shape across all members, tiles chosen by the autotuner."""
+    import numpy as np
+    from solar_amd import Config, Ensemble, background, generate_ic
+    rng = np.random.default_rng(20240)
+    spec = [(int(rng.choice([32, 48, 64, 96])), int(rng.choice([64, 128, 192])), float(rng.uniform(0.02, 0.08)),
+             int(rng.integers(1, 2**31))) for _ in range(48)]
+    members = []
+    for n, nr, width, seed in spec:
+        cfg = Config("ens", n, nr, 0, 0, steps)
+        gr, c2, rho0, g0 = background(0, nr)
+        members.append(dict(n=n, nr=nr, gr=gr, c2=c2, rho0=rho0, g0=g0, sponge_len=cfg.sponge_len,
+                            a_sponge=cfg.a_sponge, dt=cfg.dt(gr, c2)))
+    with Ensemble(members) as ens:
+        t = time.perf_counter()
+        ens.autotune(reps)
+        tune_s = time.perf_counter() - t
+        for k, (n, nr, width, seed) in enumerate(spec):
+            q0, _ = generate_ic(0, n, nr, members[k]["gr"], members[k]["c2"], seed, [1e-3, width, 0.0])
+            ens.set_state(k, q0)
+            del q0
+        ens.rk4(3)
+        ens.sync()
+        t = time.perf_counter()
+        ens.rk4_async(steps)
+        ens.sync()
+        wall = time.perf_counter() - t
+        info = ens.info()
+        small = min(range(48), key=lambda k: spec[k][0] ** 2 * spec[k][1])
+        finite = bool(np.isfinite(ens.get_state(small)).all())
+        tiles = sorted({(spec[k][0], spec[k][1]) + ens.tiles(k) for k in range(48)})
+    rate = info["points"] * steps / wall
+    return {"workload": "48 Gaussian-pulse simulations, n in {32,48,64,96}, nr in {64,128,192}, FP64",
+            "members": 48, "points": info["points"], "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "value": rate, "unit": "grid-point RK4 steps/s", "step_frac": rate * STEP_BYTES / HBM_PEAK,
+            "launches_per_step": info["launches_per_step"], "autotune_s": tune_s, "state_finite": finite,
+            "tiles_by_class": [{"n": c[0], "nr": c[1], "stage": [c[2], c[3], c[4]], "filter": [c[5], c[6], c[7]]}
+                               for c in tiles]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +171,8 @@ def main():
     ap.add_argument("--state-bits", type=int, default=64, choices=(64, 32),
                     help="storage of the state and 2N accumulator (arithmetic is FP64 either way)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint cost measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference gradient check")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run STEPS steps in FP64 and FP32 storage and report the relative L2 error "
@@ -195,6 +241,10 @@ def main():
             ["rho", "v_xi", "v_eta", "v_r", "p"], errs))}
         del fin
 
+    ensemble = None
+    if not args.no_ensemble and bits == 64:
+        ensemble = _ensemble_bench(args.ensemble_steps, 3)
+
     adjoint = None
     if not args.no_adjoint and bits == 64:
         adjoint = _adjoint_bench(not args.no_adjoint_fd)
@@ -213,6 +263,8 @@ def main():
         out["precision"] = precision
     if adjoint:
         out["adjoint"] = adjoint
+    if ensemble:
+        out["ensemble"] = ensemble
     if cpu:
         out["gpu_over_cpu"] = rate / cpu["value"]
     print(json.dumps(out), flush=True)

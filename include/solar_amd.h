@@ -173,6 +173,85 @@ int solar_adj_step(solar_ctx* ctx, double dt, const double* lam, double* out);
 int solar_misfit_gradient(solar_ctx* ctx, const double* q0, int nsteps, double dt, const double* data,
                           int checkpoint_every, double* J, double* grad, double* seis, int* segment_used);
 
+/* Step-adjacent output in the reference's on-disk formats. ------------------
+ * The reference runs one MPI rank per panel at px = py = pr = 1: rank = panel
+ * 0..5, zoneid = rank + 1 (init_mpi.f90:34-37).  Paths are relative to dir
+ * (NULL or "" = the working directory, as the reference); missing directories
+ * are created.  <ddd> = digits cue/100, (cue%100)/10, cue%10 (output.f90:121-127).
+ *
+ * WRITE_RESTART (output.f90:86-190): Restart/restart.<ddd>.<zoneid>, per zone
+ * int32 nx, ny, nr, 5; real*8 time; int32 step; then q(nx,ny,nr,5) real*8 in
+ * Fortran order from byte 28.  solar_read_restart is its inverse (the
+ * reference has none) and checks the sizes.  WRITE_VISUAL (output.f90:192-240):
+ * Visual/<ddd>/vis_<rank>, one unformatted sequential record (int32 length
+ * markers) of q(:,:,o_rad,1:5), o_rad 1-based.  WRITE_RADIAL_TRACE
+ * (output.f90:312-343): appends ((q(1,1,k,l), k=1..nr), l=1..5) to
+ * trace_<zoneid> in format (1x, 5(ES13.5)).  The handle versions write the
+ * device state (FP32 storage is widened to real*8). */
+int solar_write_restart(solar_ctx* ctx, const char* dir, int cue, double time, int step);
+int solar_read_restart(solar_ctx* ctx, const char* dir, int cue, double* time, int* step);
+int solar_write_visual(solar_ctx* ctx, const char* dir, int cue, int o_rad);
+int solar_write_radial_trace(solar_ctx* ctx, const char* dir);
+
+/* The same on host arrays q(n,n,nr,5,6) (no device needed). */
+int solar_restart_write(const char* dir, int n, int nr, const double* q, int cue, double time, int step);
+int solar_restart_read(const char* dir, int n, int nr, double* q, int cue, double* time, int* step);
+int solar_visual_write(const char* dir, int n, int nr, const double* q, int cue, int o_rad);
+int solar_radial_trace_write(const char* dir, int n, int nr, const double* q);
+/* WRITE_GRID (output.f90:242-267): Visual/grid/grid_<rank>, (1x, 7i5). */
+int solar_grid_write(const char* dir, int n, int nr);
+/* o_rad of SOLAR_DATA (ic_acoustic_wave.f90:128-133): the 1-based radial
+ * index closest to obsheight, ties to the larger index. */
+int solar_output_radius(int nr, const double* gr, double obsheight, int* o_rad);
+
+/* The norm diagnostic of cube.f90:77 for field 0..4 of the device state:
+ * reference_formula != 0 gives auxiliary.f90:100-125 as written,
+ * (sum over all panels of f^2 / (nx*ny*nr) * 6)**0.5; 0 gives the RMS over
+ * the 6 panels (solar_norm_vr for field 3). */
+int solar_norm(solar_ctx* ctx, int field, int reference_formula, double* out);
+
+/* Batched ensembles. -------------------------------------------------------
+ * An ensemble holds independent simulations with uneven grids (each member is
+ * one run of the reference's cube.f90 time loop, cube.f90:71-90, with its own
+ * n, nr, background, sponge and dt) and steps all of them together: one RK4
+ * step is one hipGraph replay of a fixed number of launches, each covering
+ * every member (a tile work list per tile shape).  FP64 state. */
+typedef struct solar_ens solar_ens;
+
+/* nmembers members; n[m], nr[m] as solar_init; gr, c2, rho0, g0 are the
+ * members' radial profiles laid end to end (member m's start at the sum of
+ * nr over members < m); sponge_len[m], a_sponge[m]; dt[m] > 0 the member's
+ * time step.  States start at zero. */
+int solar_ens_create(solar_ens** ens, int nmembers, const int* n, const int* nr, const double* gr, const double* c2,
+                     const double* rho0, const double* g0, const int* sponge_len, const double* a_sponge,
+                     const double* dt);
+int solar_ens_finalize(solar_ens* ens);
+
+/* Times the candidate tile shapes and radial chunk lengths of the stage and
+ * filter kernels for every class of members with equal (n, nr), reps
+ * launches each, keeps the fastest, then picks one block per work item or a
+ * persistent grid from the time of a whole step.  Clears every state: call it
+ * before solar_ens_set_state. */
+int solar_ens_autotune(solar_ens* ens, int reps);
+
+/* Launch parameters of member m: six[] = stage TI, TJ, planes per chunk,
+ * filter TI, TJ, planes per chunk.  set: stage shapes 64x8, 32x8, 16x16 and
+ * filter shapes 32x16, 16x16, 32x8 dividing n. */
+int solar_ens_tiles(solar_ens* ens, int m, int* six);
+int solar_ens_set_tiles(solar_ens* ens, int m, const int* six);
+
+/* State of member m, q(n,n,nr,5,6) as for solar_set_state. */
+int solar_ens_set_state(solar_ens* ens, int m, const double* q);
+int solar_ens_get_state(solar_ens* ens, int m, double* q);
+
+/* nsteps RK4 steps of every member (each with its own dt). */
+int solar_ens_rk4(solar_ens* ens, int nsteps);
+int solar_ens_rk4_async(solar_ens* ens, int nsteps);
+int solar_ens_sync(solar_ens* ens);
+
+/* Grid points over all members, launches per step, device bytes held. */
+int solar_ens_info(solar_ens* ens, size_t* points, int* launches_per_step, size_t* device_bytes);
+
 /* Description of the last error on this thread ("" if none). */
 const char* solar_last_error(void);
 

@@ -13,10 +13,10 @@ reference's Fortran interface for that path:
 There is no CPU fallback: if the library is missing or no GPU is present the
 calls raise.
 """
-from .lib import (SolarError, Solver, background, generate_ic, host_tables, load_library,
+from .lib import (SolarError, Solver, Ensemble, background, generate_ic, host_tables, load_library,
                   panel_points, RHS_BC, RHS_SPONGE, ALPHA_RK4, BETA_RK4)
 from .config import CONFIGS, Config, make_inputs
 
-__all__ = ["SolarError", "Solver", "background", "generate_ic", "host_tables", "load_library",
+__all__ = ["SolarError", "Solver", "Ensemble", "background", "generate_ic", "host_tables", "load_library",
            "panel_points", "RHS_BC", "RHS_SPONGE", "ALPHA_RK4", "BETA_RK4", "CONFIGS", "Config",
            "make_inputs"]

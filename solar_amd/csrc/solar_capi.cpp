@@ -9,10 +9,12 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/solar_amd.h"
 #include "solar_adjoint.hpp"
+#include "solar_internal.hpp"
 #include "solar_kernels.hpp"
 #include "solar_tables.hpp"
 
@@ -88,16 +90,6 @@ int dev_alloc(solar_ctx* c, T** p, size_t count) {
   return SOLAR_OK;
 }
 
-template <typename T>
-int upload(solar_ctx* c, const T** dst, const std::vector<T>& src) {
-  T* d = nullptr;
-  int rc = dev_alloc(c, &d, src.size());
-  if (rc) return rc;
-  HIPCHK(hipMemcpy(d, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-  *dst = d;
-  return SOLAR_OK;
-}
-
 void release(solar_ctx* c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
@@ -167,9 +159,49 @@ int env_int(const char* name) {
 
 }  // namespace
 
+int solar::set_error(int code, const std::string& msg) { return fail(code, msg); }
+void solar::clear_error() { g_err.clear(); }
+
+int solar::upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allocs, size_t* bytes) {
+  auto put = [&](auto** dst, const auto& src) -> int {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(src[0])>>;
+    void* v = nullptr;
+    const size_t b = src.size() * sizeof(T);
+    hipError_t e = hipMalloc(&v, b);
+    if (e != hipSuccess) return fail(SOLAR_ERR_HIP, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+    allocs->push_back(v);
+    *bytes += b;
+    e = hipMemcpy(v, src.data(), b, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    *dst = static_cast<T*>(v);
+    return SOLAR_OK;
+  };
+  int rc;
+  d->n = t.n;
+  d->nr = t.nr;
+  if ((rc = put(&d->ci, t.metric_c)) || (rc = put(&d->dj, t.metric_d)) || (rc = put(&d->dc, t.dc)) ||
+      (rc = put(&d->tdx, t.tdx)) || (rc = put(&d->tde, t.tde)) || (rc = put(&d->xy, t.xy)) ||
+      (rc = put(&d->rad, t.radial)) || (rc = put(&d->conv, t.conv)) || (rc = put(&d->gw, t.gather_w)) ||
+      (rc = put(&d->trans, t.trans)))
+    return rc;
+  std::vector<Route> routes(24);
+  for (int p = 0; p < 6; ++p)
+    for (int e = 0; e < 4; ++e) routes[p * 4 + e] = t.route[p][e];
+  if ((rc = put(&d->route, routes))) return rc;
+  for (int q = 0; q < 7; ++q) {
+    d->fcx[q] = t.fcx[q];
+    d->fcy[q] = t.fcy[q];
+    d->filt[q] = solar::kFiltCoeffs[q];
+  }
+  for (int q = 0; q < 11; ++q) d->filtr[q] = solar::kFiltCoeffsR[q];
+  return SOLAR_OK;
+}
+
 extern "C" {
 
 const char* solar_last_error(void) { return g_err.c_str(); }
+
+
 
 int solar_init(solar_ctx** out, int n, int nr, const double* gr, const double* c2, const double* rho0,
                const double* g0, int sponge_len, double a_sponge) {
@@ -207,25 +239,7 @@ int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double
   };
   if (hipStreamCreate(&c->stream) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipStreamCreate failed"));
 
-  const Tables& t = c->tab;
-  DevTables& d = c->dt;
-  d.n = n;
-  d.nr = nr;
-  if ((rc = upload(c, &d.ci, t.metric_c)) || (rc = upload(c, &d.dj, t.metric_d)) || (rc = upload(c, &d.dc, t.dc)) ||
-      (rc = upload(c, &d.tdx, t.tdx)) || (rc = upload(c, &d.tde, t.tde)) || (rc = upload(c, &d.xy, t.xy)) ||
-      (rc = upload(c, &d.rad, t.radial)) || (rc = upload(c, &d.conv, t.conv)) || (rc = upload(c, &d.gw, t.gather_w)) ||
-      (rc = upload(c, &d.trans, t.trans)))
-    return bail(rc);
-  std::vector<Route> routes(24);
-  for (int p = 0; p < 6; ++p)
-    for (int e = 0; e < 4; ++e) routes[p * 4 + e] = t.route[p][e];
-  if ((rc = upload(c, &d.route, routes))) return bail(rc);
-  for (int q = 0; q < 7; ++q) {
-    d.fcx[q] = t.fcx[q];
-    d.fcy[q] = t.fcy[q];
-    d.filt[q] = solar::kFiltCoeffs[q];
-  }
-  for (int q = 0; q < 11; ++q) d.filtr[q] = solar::kFiltCoeffsR[q];
+  if ((rc = solar::upload_tables(c->tab, &c->dt, &c->allocs, &c->bytes))) return bail(rc);
 
   const size_t nq = c->nf * 30;
   const size_t ng = static_cast<size_t>(12) * nr * n * 3;  // one field's ghosts, 6 panels x 2 sides
@@ -464,6 +478,48 @@ int solar_norm_vr(solar_ctx* c, double* rms) {
   HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   *rms = std::sqrt(h[1] / static_cast<double>(c->nf * 6));
+  return SOLAR_OK;
+}
+
+int solar_write_restart(solar_ctx* c, const char* dir, int cue, double time, int step) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_write_restart: null handle");
+  std::vector<double> h(c->nf * 30);
+  int rc = download_state(c, h.data(), c->q[0], h.size());
+  return rc ? rc : solar_restart_write(dir, c->n, c->nr, h.data(), cue, time, step);
+}
+
+int solar_read_restart(solar_ctx* c, const char* dir, int cue, double* time, int* step) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_read_restart: null handle");
+  std::vector<double> h(c->nf * 30);
+  int rc = solar_restart_read(dir, c->n, c->nr, h.data(), cue, time, step);
+  if (!rc) rc = upload_state(c, c->q[0], h.data(), h.size());
+  if (!rc) HIPCHK(hipStreamSynchronize(c->stream));  // h is released on return
+  return rc;
+}
+
+int solar_write_visual(solar_ctx* c, const char* dir, int cue, int o_rad) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_write_visual: null handle");
+  std::vector<double> h(c->nf * 30);
+  int rc = download_state(c, h.data(), c->q[0], h.size());
+  return rc ? rc : solar_visual_write(dir, c->n, c->nr, h.data(), cue, o_rad);
+}
+
+int solar_write_radial_trace(solar_ctx* c, const char* dir) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_write_radial_trace: null handle");
+  std::vector<double> h(c->nf * 30);
+  int rc = download_state(c, h.data(), c->q[0], h.size());
+  return rc ? rc : solar_radial_trace_write(dir, c->n, c->nr, h.data());
+}
+
+int solar_norm(solar_ctx* c, int field, int reference_formula, double* out) {
+  if (!c || !out || field < 0 || field > 4) return fail(SOLAR_ERR_ARG, "solar_norm: invalid argument");
+  HIPCHK(solar::launch_scan(c->q[0], c->f32, c->nf, c->acc, 1, c->stream, field));
+  double h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->acc, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const double per_panel = static_cast<double>(c->n) * c->n * c->nr;
+  // auxiliary.f90:122: (sum / (nx*ny*nr) * 6)**0.5 ; the RMS divides by the 6 panels instead
+  *out = reference_formula ? std::pow((h[1] / per_panel) * 6.0, 0.5) : std::sqrt(h[1] / (per_panel * 6.0));
   return SOLAR_OK;
 }
 

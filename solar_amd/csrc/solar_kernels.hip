@@ -65,18 +65,16 @@ struct Src {
 // for vectors, applies the receiver-basis transform (transform.f90:161-274).
 // blockIdx.y selects a field offset (the filter's five scalars).
 //   gx*: [P][side W/E][k][m][l],  gy*: [P][side S/N][k][l][m]
+//   edge = (plane k)*24 + (receiver panel P)*4 + (receiver edge e)
 template <bool VEC, typename S>
-__global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
-                                                double* __restrict__ gxs, double* __restrict__ gys,
-                                                double* __restrict__ gxv, double* __restrict__ gyv,
-                                                long long field_stride_out) {
-  extern __shared__ double sline[];  // [3 layers][3 quantities][n]: scalar, theta, phi
+__device__ __forceinline__ void gather_edge(const DevTables& T, int edge, int fld, const S* __restrict__ q,
+                                            int sfield, double* __restrict__ gxs, double* __restrict__ gys,
+                                            double* __restrict__ gxv, double* __restrict__ gyv,
+                                            long long field_stride_out, double* sline) {
   const int n = T.n, nr = T.nr;
-  const int edge = blockIdx.x;
   const int k = edge / 24;
   const int P = (edge % 24) / 4;
   const int e = edge % 4;
-  const int fld = blockIdx.y;
 
   const Route rt = T.route[P * 4 + e];
   const int s = rt.sender, se = rt.sender_edge;
@@ -138,14 +136,27 @@ __global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict
   }
 }
 
+template <bool VEC, typename S>
+__global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
+                                                double* __restrict__ gxs, double* __restrict__ gys,
+                                                double* __restrict__ gxv, double* __restrict__ gyv,
+                                                long long field_stride_out) {
+  extern __shared__ double sline[];  // [3 layers][3 quantities][n]: scalar, theta, phi
+  gather_edge<VEC, S>(T, blockIdx.x, blockIdx.y, q, sfield, gxs, gys, gxv, gyv, field_stride_out, sline);
+}
+
 // Values one thread loads for one radial plane of the stage kernel: its own
 // point, the halo cells it fills, and the plane entering the radial window.
+// State values stay in the storage type S until used (FP32 storage halves
+// the registers of the prefetched plane); halo cells may come from the FP64
+// ghost strips and are held as double.
+template <typename S>
 struct StagePlane {
-  double vx, ve, q1, pc, rc;   // v_xi, v_eta, rho', p', v_r at (i,j,k)
-  double w[5];                 // qw at (i,j,k)
+  S vx, ve, q1, pc, rc;        // v_xi, v_eta, rho', p', v_r at (i,j,k)
+  S w[5];                      // qw at (i,j,k)
   double hpx, hvx;             // i-halo cell (p', v_xi)
   double hpy0, hve0, hpy1, hve1;  // low / high j-halo cells (p', v_eta)
-  double pn, rn;               // p', v_r of the plane entering the window
+  S pn, rn;                    // p', v_r of the plane entering the window
 };
 
 // One RK stage (or one RHS evaluation when rhs_mode is set), one pass over
@@ -165,34 +176,61 @@ struct StagePlane {
 #define SOLAR_STAGE_ATTR
 #endif
 
-template <int TI, int TJ, int KIND, typename S>
-__global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, const S* __restrict__ qin,
-                                                   S* __restrict__ qout, S* __restrict__ qw,
-                                                   const double* __restrict__ gxp, const double* __restrict__ gyp,
-                                                   const double* __restrict__ gxv, const double* __restrict__ gyv,
-                                                   double alpha, double beta, double dt, int kchunk, int nkch,
+//   ENS: ensemble launch.  The block walks a list of work items (member,
+//   tile, panel, plane range) with a grid-sized stride; tables, buffers and
+//   dt come from the member (the state ping-pongs between M.q[cur] and
+//   M.q[1-cur]); the single-grid arguments are unused.
+template <int TI, int TJ, int KIND, typename S, bool ENS = false>
+__global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Targ, const S* __restrict__ qin_a,
+                                                   S* __restrict__ qout_a, S* __restrict__ qw_a,
+                                                   const double* __restrict__ gxp_a, const double* __restrict__ gyp_a,
+                                                   const double* __restrict__ gxv_a, const double* __restrict__ gyv_a,
+                                                   double alpha, double beta, double dt_a, int kchunk, int nkch,
                                                    int rhs_mode, const double* __restrict__ c2f,
-                                                   double* __restrict__ divout) {
+                                                   double* __restrict__ divout, const EnsMember* __restrict__ M,
+                                                   const int4* __restrict__ items, int nitems, int cur) {
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
   __shared__ double sve[2][TJ + 6][TI];
 
+  const int n_items = ENS ? nitems : 1;
+  for (int item = ENS ? (int)blockIdx.x : 0; item < n_items; item += ENS ? (int)gridDim.x : 1) {
+  int bi, bj, P, k0, k1, mem = 0;
+  if constexpr (ENS) {
+    const int4 w = items[item];
+    mem = w.x;
+    bi = w.y & 0xffff;
+    bj = w.y >> 16;
+    P = w.z & 0xff;
+    k0 = w.z >> 8;
+    k1 = w.w;
+  } else {
+    const int nti = Targ.n / TI, ntj = Targ.n / TJ;
+    const int nb = nti * ntj * 6 * nkch;
+    int b = xcd_remap(blockIdx.x, nb);
+    bi = b % nti;
+    b /= nti;
+    bj = b % ntj;
+    b /= ntj;
+    P = b % 6;
+    const int ch = b / 6;
+    k0 = ch * kchunk;
+    k1 = min(k0 + kchunk, Targ.nr);
+    if (k0 >= Targ.nr) return;
+  }
+  const DevTables& T = ENS ? M[mem].T : Targ;
+  const S* __restrict__ qin = ENS ? reinterpret_cast<const S*>(M[mem].q[cur]) : qin_a;
+  S* __restrict__ qout = ENS ? reinterpret_cast<S*>(M[mem].q[1 - cur]) : qout_a;
+  S* __restrict__ qw = ENS ? reinterpret_cast<S*>(M[mem].qw) : qw_a;
+  const double* __restrict__ gxp = ENS ? M[mem].gxp : gxp_a;
+  const double* __restrict__ gyp = ENS ? M[mem].gyp : gyp_a;
+  const double* __restrict__ gxv = ENS ? M[mem].gxv : gxv_a;
+  const double* __restrict__ gyv = ENS ? M[mem].gyv : gyv_a;
+  const double dt = ENS ? M[mem].dt : dt_a;
   const int n = T.n, nr = T.nr;
-  const int nti = n / TI, ntj = n / TJ;
-  const int nb = nti * ntj * 6 * nkch;
-  int b = xcd_remap(blockIdx.x, nb);
-  const int bi = b % nti;
-  b /= nti;
-  const int bj = b % ntj;
-  b /= ntj;
-  const int P = b % 6;
-  const int ch = b / 6;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = ch * kchunk;
-  const int k1 = min(k0 + kchunk, nr);
-  if (k0 >= nr) return;
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t ij = (size_t)j * n + i;
@@ -248,23 +286,23 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
     }
   }
 
-  auto load_plane = [&](int k, StagePlane& L) {
+  auto load_plane = [&](int k, StagePlane<S>& L) {
     const size_t kp = (size_t)k * plane + ij;
-    L.vx = static_cast<double>(f1[kp]);
-    L.ve = static_cast<double>(f2[kp]);
-    L.q1 = static_cast<double>(__builtin_nontemporal_load(f0 + kp));
+    L.vx = f1[kp];
+    L.ve = f2[kp];
+    L.q1 = __builtin_nontemporal_load(f0 + kp);
     // p' and v_r at plane k are the window centre on interior planes; the
     // three end planes on each side read them here.
     if (k < 3 || k >= nr - 3) {
-      L.pc = static_cast<double>(f4[kp]);
-      L.rc = static_cast<double>(f3[kp]);
+      L.pc = f4[kp];
+      L.rc = f3[kp];
     } else {
-      L.pc = 0.0;
-      L.rc = 0.0;
+      L.pc = S(0);
+      L.rc = S(0);
     }
 #pragma unroll
     for (int f = 0; f < 5; ++f)
-      L.w[f] = (KIND == 0) ? 0.0 : static_cast<double>(__builtin_nontemporal_load(w0 + f * nf + kp));
+      L.w[f] = (KIND == 0) ? S(0) : __builtin_nontemporal_load(w0 + f * nf + kp);
     const size_t kf = (size_t)k * plane, kg = (size_t)k * 3 * n;
     if (hx) {
       L.hpx = hx_g ? gxpW[kg + hx_off] : static_cast<double>(f4[kf + hx_off]);
@@ -280,8 +318,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
     }
     // plane entering the radial window when the window slides before k
     const int kn = min(k + 3, nr - 1);
-    L.pn = static_cast<double>(f4[(size_t)kn * plane + ij]);
-    L.rn = static_cast<double>(f3[(size_t)kn * plane + ij]);
+    L.pn = f4[(size_t)kn * plane + ij];
+    L.rn = f3[(size_t)kn * plane + ij];
   };
 
   const double Ci = T.ci[i], Dj = T.dj[j];
@@ -295,11 +333,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
     pw[q] = static_cast<double>(f4[(size_t)(kb + q) * plane + ij]);
     rw[q] = static_cast<double>(f3[(size_t)(kb + q) * plane + ij]);
   }
-  StagePlane nxt;
+  StagePlane<S> nxt;
   load_plane(k0, nxt);
 
   for (int k = k0; k < k1; ++k) {
-    const StagePlane cur = nxt;
+    const StagePlane<S> cur = nxt;
     if (k + 1 < k1) load_plane(k + 1, nxt);
     const int want = clampi(k - 3, 0, nr - 7);
     if (want != kb) {  // slides by exactly one plane: plane k+3 enters
@@ -398,6 +436,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
       }
     }
   }
+  if constexpr (ENS) __syncthreads();  // the next item rewrites the LDS tiles
+  }
 }
 
 // FILTER_EXP for one field of one panel over a chunk of radial planes.
@@ -407,33 +447,28 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables T, 
 // (prefetched into registers one plane ahead), FILTER_XI runs on the in-panel
 // rows, FILTER_ETA on the result, and FILTER_R from an 11-plane register
 // window.
+template <int TI, int TJ>
+struct FilterLds {
+  double sq[TJ + 6][TI + 6];
+  double sx[TJ + 6][TI];
+};
+
+// The tile (bi, bj) of field F of panel P (PF = 5P + F) over planes [k0, k1).
 template <int TI, int TJ, typename S>
-__global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restrict__ qin,
-                                                    S* __restrict__ qout, const double* __restrict__ fgx,
-                                                    const double* __restrict__ fgy, int kchunk, int nkch) {
+__device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, int PF, int k0, int k1,
+                                            const S* __restrict__ qin, S* __restrict__ qout,
+                                            const double* __restrict__ fgx, const double* __restrict__ fgy,
+                                            FilterLds<TI, TJ>& lds) {
   constexpr int NT = TI * TJ;
   constexpr int NCELL = (TJ + 6) * (TI + 6);
   constexpr int NC = (NCELL + NT - 1) / NT;
-  __shared__ double sq[TJ + 6][TI + 6];
-  __shared__ double sx[TJ + 6][TI];
-
+  auto& sq = lds.sq;
+  auto& sx = lds.sx;
   const int n = T.n, nr = T.nr;
-  const int nti = n / TI, ntj = n / TJ;
-  const int nb = nti * ntj * 30 * nkch;
-  int b = xcd_remap(blockIdx.x, nb);
-  const int bi = b % nti;
-  b /= nti;
-  const int bj = b % ntj;
-  b /= ntj;
-  const int PF = b % 30;  // panel*5 + field
-  const int ch = b / 30;
   const int P = PF / 5, F = PF % 5;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
-  const int k0 = ch * kchunk;
-  const int k1 = min(k0 + kchunk, nr);
-  if (k0 >= nr) return;
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const S* g = qin + (size_t)PF * nf;
@@ -534,19 +569,78 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restr
   }
 }
 
+template <int TI, int TJ, typename S>
+__global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restrict__ qin,
+                                                    S* __restrict__ qout, const double* __restrict__ fgx,
+                                                    const double* __restrict__ fgy, int kchunk, int nkch) {
+  __shared__ FilterLds<TI, TJ> lds;
+  const int n = T.n, nr = T.nr;
+  const int nti = n / TI, ntj = n / TJ;
+  const int nb = nti * ntj * 30 * nkch;
+  int b = xcd_remap(blockIdx.x, nb);
+  const int bi = b % nti;
+  b /= nti;
+  const int bj = b % ntj;
+  b /= ntj;
+  const int PF = b % 30;  // panel*5 + field
+  const int ch = b / 30;
+  const int k0 = ch * kchunk;
+  const int k1 = min(k0 + kchunk, nr);
+  if (k0 >= nr) return;
+  filter_tile<TI, TJ, S>(T, bi, bj, PF, k0, k1, qin, qout, fgx, fgy, lds);
+}
+
+// Ensemble FILTER_EXP: work items (member, tile, 5P+F, plane range), q[cur] -> q[1-cur].
+template <int TI, int TJ>
+__global__ __launch_bounds__(TI* TJ) void k_filter_ens(const EnsMember* __restrict__ M, const int4* __restrict__ items,
+                                                        int nitems, int cur) {
+  __shared__ FilterLds<TI, TJ> lds;
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    const int4 w = items[item];
+    const EnsMember& E = M[w.x];
+    filter_tile<TI, TJ, double>(E.T, w.y & 0xffff, w.y >> 16, w.z & 0xff, w.z >> 8, w.w, E.q[cur], E.q[1 - cur],
+                                E.fgx, E.fgy, lds);
+    __syncthreads();  // the next item rewrites the LDS tiles
+  }
+}
+
+// Ensemble gather: the blocks of all members laid end to end (prefix[m] =
+// first block of member m, 24*nr blocks each); blockIdx.y selects the field.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_gather_ens(const EnsMember* __restrict__ M, const int* __restrict__ prefix,
+                                                    int nmem, int cur) {
+  extern __shared__ double sline[];
+  const int b = blockIdx.x;
+  int lo = 0, hi = nmem - 1;  // last member whose first block is <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const EnsMember& E = M[lo];
+  const long long fs = 12LL * E.T.nr * E.T.n * 3;
+  if (VEC)
+    gather_edge<true, double>(E.T, b - prefix[lo], blockIdx.y, E.q[cur], 4, E.gxp, E.gyp, E.gxv, E.gyv, fs, sline);
+  else
+    gather_edge<false, double>(E.T, b - prefix[lo], blockIdx.y, E.q[cur], 0, E.fgx, E.fgy, nullptr, nullptr, fs,
+                               sline);
+}
+
 // Non-finite scan and v_r sum of squares; acc[0] counts non-finite values,
 // acc[1] accumulates squares.
 template <typename S>
-__global__ void k_scan(const S* __restrict__ q, size_t total, size_t nf, double* acc, int mode) {
+__global__ void k_scan(const S* __restrict__ q, size_t total, size_t nf, double* acc, int mode, int field) {
   double s = 0.0;
   for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
     if (mode == 0) {
       const double v = static_cast<double>(q[x]);
       s += (isfinite(v) ? 0.0 : 1.0);
     } else {
-      // v_r of panel p: field 3
+      // `field` of panel p
       const size_t p = x / nf, r = x % nf;
-      const double v = static_cast<double>(q[(p * 5 + 3) * nf + r]);
+      const double v = static_cast<double>(q[(p * 5 + field) * nf + r]);
       s += v * v;
     }
   }
@@ -614,13 +708,13 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   if (rhs_mode || stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout);
+                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout);
+                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout);
+                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
   return hipGetLastError();
 }
 
@@ -705,14 +799,16 @@ hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f
   return launch_filter_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), fgx, fgy, st, ls);
 }
 
-hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st) {
+hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st, int field) {
+  if (field < 0 || field > 4) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, acc, 2);
   const size_t total = mode == 0 ? nf * 30 : nf * 6;
   if (f32)
-    hipLaunchKernelGGL(k_scan<float>, dim3(1024), dim3(256), 0, st, static_cast<const float*>(q), total, nf, acc, mode);
+    hipLaunchKernelGGL(k_scan<float>, dim3(1024), dim3(256), 0, st, static_cast<const float*>(q), total, nf, acc, mode,
+                       field);
   else
     hipLaunchKernelGGL(k_scan<double>, dim3(1024), dim3(256), 0, st, static_cast<const double*>(q), total, nf, acc,
-                       mode);
+                       mode, field);
   return hipGetLastError();
 }
 
@@ -721,6 +817,62 @@ hipError_t launch_place(const double* src, void* dst, bool f32, size_t nf, int d
     hipLaunchKernelGGL(k_place<float>, dim3(1024), dim3(256), 0, st, src, static_cast<float*>(dst), nf, dfield);
   else
     hipLaunchKernelGGL(k_place<double>, dim3(1024), dim3(256), 0, st, src, static_cast<double*>(dst), nf, dfield);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- ensembles
+
+template <int TI, int TJ>
+static hipError_t launch_stage_ens_t(const EnsMember* M, const int4* items, int nitems, int grid, int stage, int cur,
+                                     hipStream_t st) {
+  const double al = kAlphaRk4[stage], be = kBetaRk4[stage];
+  DevTables none{};
+  dim3 g(grid), block(TI, TJ);
+  if (stage == 0)
+    hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+  else if (stage < 4)
+    hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+  else
+    hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+  return hipGetLastError();
+}
+
+hipError_t launch_stage_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int stage,
+                            int cur, hipStream_t st) {
+  const int4* it = reinterpret_cast<const int4*>(items);
+  if (nitems <= 0) return hipSuccess;
+  if (stage < 0 || stage > 4) return hipErrorInvalidValue;
+  if (ti == 64 && tj == 8) return launch_stage_ens_t<64, 8>(M, it, nitems, grid, stage, cur, st);
+  if (ti == 32 && tj == 8) return launch_stage_ens_t<32, 8>(M, it, nitems, grid, stage, cur, st);
+  if (ti == 16 && tj == 16) return launch_stage_ens_t<16, 16>(M, it, nitems, grid, stage, cur, st);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int cur,
+                             hipStream_t st) {
+  const int4* it = reinterpret_cast<const int4*>(items);
+  if (nitems <= 0) return hipSuccess;
+  if (ti == 32 && tj == 16)
+    hipLaunchKernelGGL((k_filter_ens<32, 16>), dim3(grid), dim3(32, 16), 0, st, M, it, nitems, cur);
+  else if (ti == 16 && tj == 16)
+    hipLaunchKernelGGL((k_filter_ens<16, 16>), dim3(grid), dim3(16, 16), 0, st, M, it, nitems, cur);
+  else if (ti == 32 && tj == 8)
+    hipLaunchKernelGGL((k_filter_ens<32, 8>), dim3(grid), dim3(32, 8), 0, st, M, it, nitems, cur);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
+                             hipStream_t st) {
+  const size_t lds = sizeof(double) * 9 * (size_t)maxn;
+  if (vec)
+    hipLaunchKernelGGL(k_gather_ens<true>, dim3(nblocks, 1), dim3(256), lds, st, M, prefix, nmem, cur);
+  else
+    hipLaunchKernelGGL(k_gather_ens<false>, dim3(nblocks, 5), dim3(256), lds, st, M, prefix, nmem, cur);
   return hipGetLastError();
 }
 

@@ -24,6 +24,17 @@ struct DevTables {
   double filt[7], filtr[11];
 };
 
+// One member of a batched ensemble (FP64): its tables, state ping-pong
+// buffers, 2N accumulator, ghost strips and time step.
+struct EnsMember {
+  DevTables T;
+  double* q[2];
+  double* qw;
+  double *gxp, *gyp, *gxv, *gyv;
+  double *fgx, *fgy;
+  double dt;
+};
+
 // Launch-shape overrides (0 = automatic).
 struct LaunchShape {
   int stage_chunks = 0;
@@ -43,7 +54,22 @@ hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* q
                         double* divout = nullptr);
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
                          const double* fgy, hipStream_t st, const LaunchShape& ls);
-hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st);
+// Ensemble launches.  items: 4 ints per work item, (member, bi | bj << 16,
+// P (stage) or 5P+F (filter) | k0 << 8, k1); grid blocks walk the list with a
+// grid-sized stride.  Stage ti x tj in {64x8, 32x8, 16x16}, filter in
+// {32x16, 16x16, 32x8}.  The state goes from M.q[cur] to M.q[1-cur].
+hipError_t launch_stage_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int stage,
+                            int cur, hipStream_t st);
+hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int cur,
+                             hipStream_t st);
+// prefix[m]: first block of member m (24*nr blocks each), nblocks in total;
+// maxn: the largest n (LDS size).  vec: the stage ghosts (p', velocity),
+// else the filter's five scalars.
+hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
+                             hipStream_t st);
+// mode 0: count of non-finite values -> acc[0]; mode 1: sum of squares of
+// `field` over all panels -> acc[1].
+hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st, int field = 3);
 hipError_t launch_place(const double* src, void* dst, bool f32, size_t nf, int dfield, hipStream_t st);
 
 }  // namespace solar

@@ -19,6 +19,8 @@ _D = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
 # (name, restype, argtypes) of every entry point declared in include/solar_amd.h
+_IP = ctypes.POINTER(ctypes.c_int)
+
 EXPORTS = [
     ("solar_init", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _D, _D, _D, _D,
                                   ctypes.c_int, ctypes.c_double]),
@@ -49,6 +51,33 @@ EXPORTS = [
     ("solar_adj_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, _D, _D]),
     ("solar_misfit_gradient", ctypes.c_int, [ctypes.c_void_p, _D, ctypes.c_int, ctypes.c_double, _D, ctypes.c_int,
                                              _D, _D, _D, ctypes.POINTER(ctypes.c_int)]),
+    ("solar_write_restart", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_int]),
+    ("solar_read_restart", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _D, _IP]),
+    ("solar_write_visual", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    ("solar_write_radial_trace", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    ("solar_restart_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_int]),
+    ("solar_restart_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _D, _IP]),
+    ("solar_visual_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int,
+                                          ctypes.c_int]),
+    ("solar_radial_trace_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D]),
+    ("solar_grid_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    ("solar_output_radius", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_double, _IP]),
+    ("solar_norm", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _D]),
+    ("solar_ens_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _IP, _IP, _D, _D, _D, _D,
+                                        _IP, _D, _D]),
+    ("solar_ens_finalize", ctypes.c_int, [ctypes.c_void_p]),
+    ("solar_ens_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("solar_ens_tiles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _IP]),
+    ("solar_ens_set_tiles", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _IP]),
+    ("solar_ens_set_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("solar_ens_get_state", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D]),
+    ("solar_ens_rk4", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("solar_ens_rk4_async", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("solar_ens_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("solar_ens_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _IP,
+                                      ctypes.POINTER(ctypes.c_size_t)]),
     ("solar_last_error", ctypes.c_char_p, []),
     ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                          _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -131,6 +160,54 @@ def host_tables(n, nr, gr, c2, rho0, g0, sponge_len, a_sponge):
                                              ("scal", "n1", "r1", "n2", "interp", "ghost", "sigma", "drho0dr")]),
            "solar_host_tables")
     return out
+
+
+def _dir(d):
+    return None if d is None else str(d).encode()
+
+
+def restart_write(dirname, q, cue, time, step):
+    """WRITE_RESTART files Restart/restart.<ddd>.<zone> for a host state (6, 5, nr, n, n)."""
+    q = _f64(q)
+    _check(load_library().solar_restart_write(_dir(dirname), q.shape[-1], q.shape[2], _p(q), int(cue), float(time),
+                                              int(step)), "solar_restart_write")
+
+
+def restart_read(dirname, n, nr, cue):
+    """Inverse of restart_write: (q of shape (6, 5, nr, n, n), time, step)."""
+    q = np.empty((6, 5, nr, n, n))
+    t, st = ctypes.c_double(), ctypes.c_int()
+    _check(load_library().solar_restart_read(_dir(dirname), n, nr, _p(q), int(cue), ctypes.byref(t),
+                                             ctypes.byref(st)), "solar_restart_read")
+    return q, t.value, st.value
+
+
+def visual_write(dirname, q, cue, o_rad):
+    """WRITE_VISUAL files Visual/<ddd>/vis_<rank> (o_rad 1-based)."""
+    q = _f64(q)
+    _check(load_library().solar_visual_write(_dir(dirname), q.shape[-1], q.shape[2], _p(q), int(cue), int(o_rad)),
+           "solar_visual_write")
+
+
+def radial_trace_write(dirname, q):
+    """Appends one WRITE_RADIAL_TRACE record to trace_<zone> for every zone."""
+    q = _f64(q)
+    _check(load_library().solar_radial_trace_write(_dir(dirname), q.shape[-1], q.shape[2], _p(q)),
+           "solar_radial_trace_write")
+
+
+def grid_write(dirname, n, nr):
+    """WRITE_GRID files Visual/grid/grid_<rank>."""
+    _check(load_library().solar_grid_write(_dir(dirname), int(n), int(nr)), "solar_grid_write")
+
+
+def output_radius(gr, obsheight):
+    """o_rad (1-based) of SOLAR_DATA for an observation height."""
+    gr = _f64(gr)
+    k = ctypes.c_int()
+    _check(load_library().solar_output_radius(gr.size, _p(gr), float(obsheight), ctypes.byref(k)),
+           "solar_output_radius")
+    return k.value
 
 
 class Solver:
@@ -242,6 +319,28 @@ class Solver:
         _check(self.lib.solar_norm_vr(self.h, ctypes.byref(v)), "solar_norm_vr")
         return v.value
 
+    def norm(self, field, reference_formula=False):
+        """cube.f90:77 diagnostic of field 0..4 (auxiliary.f90:100-125 when reference_formula)."""
+        v = ctypes.c_double()
+        _check(self.lib.solar_norm(self.h, int(field), int(bool(reference_formula)), ctypes.byref(v)), "solar_norm")
+        return v.value
+
+    def write_restart(self, dirname, cue, time, step):
+        _check(self.lib.solar_write_restart(self.h, _dir(dirname), int(cue), float(time), int(step)),
+               "solar_write_restart")
+
+    def read_restart(self, dirname, cue):
+        t, st = ctypes.c_double(), ctypes.c_int()
+        _check(self.lib.solar_read_restart(self.h, _dir(dirname), int(cue), ctypes.byref(t), ctypes.byref(st)),
+               "solar_read_restart")
+        return t.value, st.value
+
+    def write_visual(self, dirname, cue, o_rad):
+        _check(self.lib.solar_write_visual(self.h, _dir(dirname), int(cue), int(o_rad)), "solar_write_visual")
+
+    def write_radial_trace(self, dirname):
+        _check(self.lib.solar_write_radial_trace(self.h, _dir(dirname)), "solar_write_radial_trace")
+
     def kernel_times(self):
         out = np.zeros(5)
         _check(self.lib.solar_kernel_times(self.h, _p(out)), "solar_kernel_times")
@@ -312,3 +411,84 @@ class Solver:
                "solar_misfit_gradient")
         self.segment_used = seg.value
         return J.value, grad, seis
+
+
+class Ensemble:
+    """Independent simulations with uneven grids stepped together.
+
+    members: sequence of dicts with n, nr, gr, c2, rho0, g0, sponge_len,
+    a_sponge, dt.  States are numpy float64 of shape (6, 5, nr, n, n)."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = [dict(m) for m in members]
+        k = len(self.members)
+        ia = lambda key: (ctypes.c_int * k)(*[int(m[key]) for m in self.members])  # noqa: E731
+        cat = lambda key: _f64(np.concatenate([_f64(m[key]) for m in self.members]))  # noqa: E731
+        self._keep = [cat(x) for x in ("gr", "c2", "rho0", "g0")]
+        asp = _f64([m.get("a_sponge", 0.1) for m in self.members])
+        dts = _f64([m["dt"] for m in self.members])
+        h = ctypes.c_void_p()
+        _check(self.lib.solar_ens_create(ctypes.byref(h), k, ia("n"), ia("nr"), *[_p(a) for a in self._keep],
+                                         ia("sponge_len"), _p(asp), _p(dts)), "solar_ens_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.solar_ens_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, m):
+        d = self.members[m]
+        return (6, 5, d["nr"], d["n"], d["n"])
+
+    def autotune(self, reps=3):
+        _check(self.lib.solar_ens_autotune(self.h, int(reps)), "solar_ens_autotune")
+
+    def tiles(self, m):
+        out = (ctypes.c_int * 6)()
+        _check(self.lib.solar_ens_tiles(self.h, int(m), out), "solar_ens_tiles")
+        return tuple(out)
+
+    def set_tiles(self, m, six):
+        arr = (ctypes.c_int * 6)(*[int(v) for v in six])
+        _check(self.lib.solar_ens_set_tiles(self.h, int(m), arr), "solar_ens_set_tiles")
+
+    def set_state(self, m, q):
+        q = _f64(q)
+        if q.size != int(np.prod(self._shape(m))):
+            raise SolarError(f"member {m}: array of {q.size} elements, expected {self._shape(m)}")
+        _check(self.lib.solar_ens_set_state(self.h, int(m), _p(q)), "solar_ens_set_state")
+
+    def get_state(self, m):
+        q = np.empty(self._shape(m))
+        _check(self.lib.solar_ens_get_state(self.h, int(m), _p(q)), "solar_ens_get_state")
+        return q
+
+    def rk4(self, nsteps):
+        _check(self.lib.solar_ens_rk4(self.h, int(nsteps)), "solar_ens_rk4")
+
+    def rk4_async(self, nsteps):
+        _check(self.lib.solar_ens_rk4_async(self.h, int(nsteps)), "solar_ens_rk4_async")
+
+    def sync(self):
+        _check(self.lib.solar_ens_sync(self.h), "solar_ens_sync")
+
+    def info(self):
+        pts, nb = ctypes.c_size_t(), ctypes.c_size_t()
+        nl = ctypes.c_int()
+        _check(self.lib.solar_ens_info(self.h, ctypes.byref(pts), ctypes.byref(nl), ctypes.byref(nb)),
+               "solar_ens_info")
+        return dict(points=pts.value, launches_per_step=nl.value, device_bytes=nb.value)

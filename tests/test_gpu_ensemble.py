@@ -1,0 +1,92 @@
+"""Batched uneven ensemble: every member, stepped together with the others in
+shared launches, must equal the same simulation run alone through the
+single-grid path bit for bit (which itself equals the CPU restatement bit for
+bit), whatever tile shapes and chunk lengths the members use; members are
+independent."""
+import numpy as np
+import pytest
+
+import oracle
+from solar_amd import Config, Ensemble, Solver, background, generate_ic
+
+pytestmark = pytest.mark.gpu
+
+# (n, nr, background kind, pulse width, seed)
+MEMBERS = [(16, 24, 0, 0.05, 1), (32, 40, 1, 0.03, 2), (48, 24, 1, 0.08, 3), (16, 36, 0, 0.02, 4),
+           (32, 24, 0, 0.06, 5), (64, 20, 1, 0.04, 6)]
+STEPS = 6
+
+
+def _members():
+    out, q0s = [], []
+    for n, nr, kind, width, seed in MEMBERS:
+        cfg = Config("m", n, nr, kind, 0, STEPS)
+        gr, c2, rho0, g0 = background(kind, nr)
+        q0, _ = generate_ic(0, n, nr, gr, c2, seed, [1e-3, width, 0.0])
+        out.append(dict(n=n, nr=nr, gr=gr, c2=c2, rho0=rho0, g0=g0, sponge_len=cfg.sponge_len,
+                        a_sponge=cfg.a_sponge, dt=cfg.dt(gr, c2)))
+        q0s.append(q0)
+    return out, q0s
+
+
+def _alone(m, q0, steps):
+    with Solver(m["n"], m["nr"], m["gr"], m["c2"], m["rho0"], m["g0"], m["sponge_len"], m["a_sponge"]) as s:
+        s.set_state(q0)
+        s.rk4(steps, m["dt"])
+        return s.get_state()
+
+
+def _run(ens, q0s, steps):
+    for k, q0 in enumerate(q0s):
+        ens.set_state(k, q0)
+    ens.rk4(steps)
+    return [ens.get_state(k) for k in range(len(q0s))]
+
+
+def test_members_match_single_grid_runs():
+    mems, q0s = _members()
+    with Ensemble(mems) as ens:
+        info = ens.info()
+        print(f"[ens] {len(mems)} members, {info['points']} points, {info['launches_per_step']} launches/step",
+              flush=True)
+        got = _run(ens, q0s, STEPS)
+    for k, (m, q0) in enumerate(zip(mems, q0s)):
+        want = _alone(m, q0, STEPS)
+        assert np.array_equal(got[k], want), f"member {k} (n={m['n']}, nr={m['nr']})"
+    # and one member against the CPU restatement directly
+    m = mems[1]
+    ref, _ = oracle.Oracle(m["n"], m["nr"], m["gr"], m["c2"], m["rho0"], m["g0"], m["sponge_len"],
+                           m["a_sponge"]).rk4(q0s[1], STEPS, m["dt"])
+    assert np.array_equal(got[1], ref)
+
+
+def test_autotuned_and_forced_tiles_give_identical_results():
+    mems, q0s = _members()
+    with Ensemble(mems) as ens:
+        base = _run(ens, q0s, 3)
+        ens.autotune(reps=2)
+        print("[ens] tuned tiles:", [ens.tiles(k) for k in range(len(mems))], flush=True)
+        tuned = _run(ens, q0s, 3)
+        # every stage/filter shape that divides n, with short and long chunks
+        for k, m in enumerate(mems):
+            n = m["n"]
+            st = [s for s in ((64, 8), (32, 8), (16, 16)) if n % s[0] == 0 and n % s[1] == 0]
+            ft = [f for f in ((32, 16), (16, 16), (32, 8)) if n % f[0] == 0 and n % f[1] == 0]
+            ens.set_tiles(k, (*st[-1], 7, *ft[-1], 11))
+        forced = _run(ens, q0s, 3)
+    for k in range(len(mems)):
+        assert np.array_equal(tuned[k], base[k]), k
+        assert np.array_equal(forced[k], base[k]), k
+
+
+def test_members_are_independent():
+    mems, q0s = _members()
+    with Ensemble(mems) as ens:
+        a = _run(ens, q0s, 2)
+        q1 = [q.copy() for q in q0s]
+        q1[2] = 2.0 * q1[2] + 1e-3
+        b = _run(ens, q1, 2)
+    for k in range(len(mems)):
+        if k != 2:
+            assert np.array_equal(a[k], b[k]), k
+    assert not np.array_equal(a[2], b[2])
