@@ -57,7 +57,7 @@ def _cpu_baseline(cfg, bg, q0, dt, sample_steps):
                       f"after one warm-up step, C restatement of the reference with OpenMP, {el:.1f} s"}
 
 
-def _adjoint_bench(fd_check):
+def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
     """Sensitivity workload: misfit of 16 p' receivers at r = 0.98 over the c5
     run (6x96x96x192, 2000 steps), data from a model whose c^2 carries a 2%
     Gaussian blob; gradient dJ/dc^2 by checkpointed discrete adjoint.  Reports
@@ -67,7 +67,8 @@ def _adjoint_bench(fd_check):
     cfg = CONFIGS["c5"]
     bg, q0, dt, _ = make_inputs(cfg)
     gr, c2 = bg[0], bg[1]
-    n, nr, steps = cfg.n, cfg.nr, cfg.steps
+    n, nr = cfg.n, cfg.nr
+    steps = steps or cfg.steps
     kr = int(np.argmin(np.abs(gr - 0.98)))
     spots = ((n // 4, n // 4), (3 * n // 4, 3 * n // 4), (n // 2, n // 5))
     rec = np.array([[p, kr, j, i] for p in range(6) for (j, i) in spots][:16])  # 16 receivers
@@ -94,7 +95,7 @@ def _adjoint_bench(fd_check):
         s.rk4(steps, dt)
         fwd = time.perf_counter() - t
         t = time.perf_counter()
-        J, grad, seis = s.misfit_gradient(q0, steps, dt, data)
+        J, grad, seis = s.misfit_gradient(q0, steps, dt, data, checkpoint_every)
         tot = time.perf_counter() - t
         out.update({"fwd_s": fwd, "fwd_adj_s": tot, "ratio": tot / fwd, "segment_steps": s.segment_used,
                     "J": J, "grad_finite": bool(np.isfinite(grad).all())})
@@ -174,6 +175,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference gradient check")
+    ap.add_argument("--adjoint-steps", type=int, default=None, help="steps of the adjoint workload (default 2000)")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="segment length of the adjoint run (0: the longest that fits in device memory)")
+    ap.add_argument("--only", choices=("adjoint", "ensemble"), default=None,
+                    help="run one extra section alone (for profiling); prints its JSON object")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run STEPS steps in FP64 and FP32 storage and report the relative L2 error "
                          "of each field of the FP32 run against the FP64 run")
@@ -182,6 +188,14 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is described in DESIGN.md, nothing runs it)", file=sys.stderr)
         sys.exit(2)
+
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": _adjoint_bench(not args.no_adjoint_fd, args.adjoint_steps,
+                                                    args.checkpoint_every)}), flush=True)
+        return
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": _ensemble_bench(args.ensemble_steps, 3)}), flush=True)
+        return
 
     from solar_amd import CONFIGS, Solver, make_inputs
     cfg = CONFIGS[args.workload]
@@ -247,7 +261,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint and bits == 64:
-        adjoint = _adjoint_bench(not args.no_adjoint_fd)
+        adjoint = _adjoint_bench(not args.no_adjoint_fd, args.adjoint_steps, args.checkpoint_every)
 
     cpu = None
     if not args.no_cpu:

@@ -23,14 +23,14 @@
 //                  filter.f90:25-28,43).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "solar_adjoint.hpp"
 #include "solar_device.hpp"
 
 namespace solar {
 
 namespace {
-
-__device__ __forceinline__ int clampi_a(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ __forceinline__ int strip_index_a(int e, int l, int a, int n) {
   switch (e) {
@@ -50,7 +50,9 @@ struct AdjU {
 };
 
 // Transposed pointwise part of COMPUTE_RHS at one point (BC_RHS^T applied to
-// W first: BC zeroes the end planes of the RHS).
+// W first: BC zeroes the end planes of the RHS).  The metric quotients come
+// from tables (the transpose needs no bit-for-bit agreement with the forward
+// expression order, only the same operator up to rounding).
 __device__ __forceinline__ void adj_point(const DevTables& T, const RadialRow& row, int k, int i, int j,
                                           double c2rho0, double w1, double w2, double w3, double w4, double w5,
                                           AdjU& u) {
@@ -58,18 +60,19 @@ __device__ __forceinline__ void adj_point(const DevTables& T, const RadialRow& r
   const double b1 = bc ? 0.0 : w1, b2 = bc ? 0.0 : w2, b3 = bc ? 0.0 : w3, b4 = bc ? 0.0 : w4,
                b5 = bc ? 0.0 : w5;
   const size_t ij = (size_t)j * T.n + i;
-  const double Ci = T.ci[i], Dj = T.dj[j], dc = T.dc[ij], xy = T.xy[ij];
+  const double Ci = T.ci[i], Dj = T.dj[j];
+  const double4 mq = *reinterpret_cast<const double4*>(T.adj_metric + ij * 4);  // dc/C, dc/D, XY/C, XY/D
   // r1 = -rho0 div - v_r drho0dr ; r5 = -c2rho0 div + rho0 g0 v_r
   const double divb = -row.rho0 * b1 - c2rho0 * b5;
   // r2..r4 = -grad/rho0
-  const double gxb = -b2 / row.rho0, geb = -b3 / row.rho0, grb = -b4 / row.rho0;
+  const double gxb = -b2 * row.inv_rho0, geb = -b3 * row.inv_rho0, grb = -b4 * row.inv_rho0;
   // div = inv_r*(dc*(Dx vxi/C + Dy veta/D) - tdx vxi - tde veta) + Dr vr + 2 vr/r
-  u.uxv = row.inv_r * dc / Ci * divb;
-  u.uyv = row.inv_r * dc / Dj * divb;
+  u.uxv = row.inv_r * mq.x * divb;
+  u.uyv = row.inv_r * mq.y * divb;
   u.urv = divb;
   // gxi = inv_r*(D Dx p + XY Dy p/D), geta = inv_r*(C Dy p + XY Dx p/C), gr = Dr p
-  u.uxp = row.inv_r * (Dj * gxb + xy / Ci * geb);
-  u.uyp = row.inv_r * (xy / Dj * gxb + Ci * geb);
+  u.uxp = row.inv_r * (Dj * gxb + mq.z * geb);
+  u.uyp = row.inv_r * (mq.w * gxb + Ci * geb);
   u.urp = grb;
   u.divb = divb;
   u.b1 = b1;
@@ -101,19 +104,23 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
                                                        double* __restrict__ gxa_v, double* __restrict__ gya_v,
                                                        double dt, double alpha, double beta, double wscale,
                                                        int kchunk, int nkch, int mode) {
-  __shared__ double sxv[TJ][TI + 6], sxp[TJ][TI + 6];
-  __shared__ double syv[TJ + 6][TI], syp[TJ + 6][TI];
+  // Transposed-stencil inputs with i halos (x) and j halos (y); two buffers
+  // alternating by plane, so one barrier per plane suffices.
+  __shared__ double sxv[2][TJ][TI + 6], sxp[2][TJ][TI + 6];
+  __shared__ double syv[2][TJ + 6][TI], syp[2][TJ + 6][TI];
+  constexpr int NT = TI * TJ, NHX = 6 * TJ, NH = 6 * TJ + 6 * TI;
+  static_assert(NH <= NT, "one halo cell per thread at most");
 
   const int n = T.n, nr = T.nr;
   const int nti = n / TI, ntj = n / TJ;
-  int b = blockIdx.x;
+  int b = xcd_remap(blockIdx.x, nti * ntj * 6 * nkch);
   const int bi = b % nti;
   b /= nti;
   const int bj = b % ntj;
   b /= ntj;
   const int P = b % 6;
   const int ch = b / 6;
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TI + tx;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
   const int k0 = ch * kchunk;
@@ -127,10 +134,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
   const double ws = W ? 1.0 : wscale;
   auto wat = [&](int f, int k, size_t cell) { return ws * w0[(size_t)f * nf + (size_t)k * plane + cell]; };
 
-  // Radial windows of the transposed-DDR inputs over planes k-3..k+3, plus the
-  // three closure rows at each shell end (their stencils reach planes 0..6 and
-  // nr-7..nr-1, derivative.f90:704-716,742-754).
-  double rv[7], rp[7], rv_lo[3], rp_lo[3], rv_hi[3], rp_hi[3];
+  // Radial inputs (u_rv, u_rp) of plane k at this column.
   auto radial_u = [&](int k, double& uv, double& up) {
     if (k < 0 || k >= nr) {
       uv = 0.0;
@@ -144,80 +148,127 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     uv = u.urv;
     up = u.urp;
   };
+  // Radial window over planes k-3..k+3.
+  double rv[7], rp[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) radial_u(k0 - 3 + q, rv[q], rp[q]);
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    rv_lo[q] = rp_lo[q] = rv_hi[q] = rp_hi[q] = 0.0;
-    if (k0 <= 6) radial_u(q, rv_lo[q], rp_lo[q]);
-    if (k1 >= nr - 7) radial_u(nr - 3 + q, rv_hi[q], rp_hi[q]);
-  }
 
-  const bool hx_lo = tx < 3, hx_hi = tx >= TI - 3, hy_lo = ty < 3, hy_hi = ty >= TJ - 3;
+  // The halo cell this thread fills (at most one per thread when NH <= NT):
+  // x halos first (TJ rows x 6 columns), then y halos (6 rows x TI columns).
+  // hcell < 0: beyond the panel edge, where the transposed stencil reads 0.
+  const bool has_h = tid < NH;
+  bool hx = false;
+  int hrow = 0, hcol = 0, hi_ = 0, hj_ = 0;
+  if (has_h) {
+    if (tid < NHX) {
+      hx = true;
+      hrow = tid / 6;
+      const int c6 = tid % 6;
+      hcol = c6 < 3 ? c6 : TI + c6;
+      hi_ = i0 - 3 + hcol;
+      hj_ = j0 + hrow;
+    } else {
+      const int h = tid - NHX;
+      const int r6 = h / TI;
+      hcol = h % TI;
+      hrow = r6 < 3 ? r6 : TJ + r6;
+      hi_ = i0 + hcol;
+      hj_ = j0 - 3 + hrow;
+    }
+  }
+  const bool h_in = has_h && hi_ >= 0 && hi_ < n && hj_ >= 0 && hj_ < n;
+  const size_t hcell = h_in ? (size_t)hj_ * n + hi_ : 0;
+
   const double sig_dt = (mode & kAdjRhsOnly) ? 1.0 : dt;
+  const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
+
+  // W values of one plane, loaded a plane ahead: own point, halo cell, and
+  // the plane entering the radial window (k+3 for plane k).
+  // Ybar, div(v) and the c^2 gradient at the own point come with them.
+  const bool upd = !(mode & kAdjRhsOnly), grad = (mode & kAdjGrad) != 0;
+  const double* yi = Yin + (size_t)(P * 5) * nf;
+  struct AdjPlane {
+    double w[5], hw[4], rw[3], y[5], dv, g;
+  };
+  auto load_plane = [&](int k, AdjPlane& L) {
+#pragma unroll
+    for (int f = 0; f < 5; ++f) L.w[f] = w0[(size_t)f * nf + (size_t)k * plane + ij];
+    if (upd) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) L.y[f] = __builtin_nontemporal_load(yi + (size_t)f * nf + (size_t)k * plane + ij);
+    }
+    if (grad) {
+      const size_t g = (size_t)P * nf + (size_t)k * plane + ij;
+      L.dv = __builtin_nontemporal_load(divs + g);
+      L.g = gc2[g];
+    }
+    if (h_in) {
+      const size_t hk = (size_t)k * plane + hcell;
+      L.hw[0] = w0[hk];
+      L.hw[1] = w0[nf + hk];
+      L.hw[2] = w0[2 * nf + hk];
+      L.hw[3] = w0[4 * nf + hk];
+    }
+    const int ke = k + 3;
+    if (k > k0 && ke < nr) {
+      const size_t ek = (size_t)ke * plane + ij;
+      L.rw[0] = w0[ek];
+      L.rw[1] = w0[3 * nf + ek];
+      L.rw[2] = w0[4 * nf + ek];
+    }
+  };
+  AdjPlane nxt;
+  load_plane(k0, nxt);
 
   for (int k = k0; k < k1; ++k) {
+    const AdjPlane cur = nxt;
+    if (k + 1 < k1) load_plane(k + 1, nxt);
+    const RadialRow& row = T.rad[k];
     if (k > k0) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
         rv[q] = rv[q + 1];
         rp[q] = rp[q + 1];
       }
-      radial_u(k + 3, rv[6], rp[6]);
+      const int ke = k + 3;
+      if (ke < nr) {
+        const RadialRow& re = T.rad[ke];
+        AdjU ue;
+        adj_point(T, re, ke, i, j, c2rho0_at(T, re, c2f, P, ke, ij), ws * cur.rw[0], 0.0, 0.0, ws * cur.rw[1],
+                  ws * cur.rw[2], ue);
+        rv[6] = ue.urv;
+        rp[6] = ue.urp;
+      } else {
+        rv[6] = 0.0;
+        rp[6] = 0.0;
+      }
     }
-    const RadialRow& row = T.rad[k];
     const size_t kp = (size_t)k * plane;
+    const int bf = (k - k0) & 1;
     // own point
-    const double w1 = wat(0, k, ij), w2 = wat(1, k, ij), w3 = wat(2, k, ij), w4 = wat(3, k, ij), w5 = wat(4, k, ij);
+    const double w1 = ws * cur.w[0], w2 = ws * cur.w[1], w3 = ws * cur.w[2], w4 = ws * cur.w[3], w5 = ws * cur.w[4];
     AdjU u;
     adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), w1, w2, w3, w4, w5, u);
-    sxv[ty][tx + 3] = u.uxv;
-    sxp[ty][tx + 3] = u.uxp;
-    syv[ty + 3][tx] = u.uyv;
-    syp[ty + 3][tx] = u.uyp;
-    // halo cells (zero beyond the panel edge: the transposed stencil has no ghosts)
-    if (hx_lo || hx_hi) {
-      const int ii = hx_lo ? i0 - 3 + tx : i + 3;
-      const int col = hx_lo ? tx : tx + 6;
-      double a = 0.0, c = 0.0;
-      if (ii >= 0 && ii < n) {
-        const size_t cell = (size_t)j * n + ii;
+    sxv[bf][ty][tx + 3] = u.uxv;
+    sxp[bf][ty][tx + 3] = u.uxp;
+    syv[bf][ty + 3][tx] = u.uyv;
+    syp[bf][ty + 3][tx] = u.uyp;
+    if (has_h) {
+      double hv = 0.0, hp = 0.0;
+      if (h_in) {
         AdjU h;
-        adj_point(T, row, k, ii, j, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
-                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
-        a = h.uxv;
-        c = h.uxp;
+        adj_point(T, row, k, hi_, hj_, c2rho0_at(T, row, c2f, P, k, hcell), ws * cur.hw[0], ws * cur.hw[1],
+                  ws * cur.hw[2], 0.0, ws * cur.hw[3], h);
+        hv = hx ? h.uxv : h.uyv;
+        hp = hx ? h.uxp : h.uyp;
       }
-      sxv[ty][col] = a;
-      sxp[ty][col] = c;
-    }
-    if (hy_lo) {
-      const int jj = j0 - 3 + ty;
-      double a = 0.0, c = 0.0;
-      if (jj >= 0) {
-        const size_t cell = (size_t)jj * n + i;
-        AdjU h;
-        adj_point(T, row, k, i, jj, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
-                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
-        a = h.uyv;
-        c = h.uyp;
+      if (hx) {
+        sxv[bf][hrow][hcol] = hv;
+        sxp[bf][hrow][hcol] = hp;
+      } else {
+        syv[bf][hrow][hcol] = hv;
+        syp[bf][hrow][hcol] = hp;
       }
-      syv[ty][tx] = a;
-      syp[ty][tx] = c;
-    }
-    if (hy_hi) {
-      const int jj = j + 3;
-      double a = 0.0, c = 0.0;
-      if (jj < n) {
-        const size_t cell = (size_t)jj * n + i;
-        AdjU h;
-        adj_point(T, row, k, i, jj, c2rho0_at(T, row, c2f, P, k, cell), wat(0, k, cell), wat(1, k, cell),
-                  wat(2, k, cell), 0.0, wat(4, k, cell), h);
-        a = h.uyv;
-        c = h.uyp;
-      }
-      syv[ty + 6][tx] = a;
-      syp[ty + 6][tx] = c;
     }
     __syncthreads();
 
@@ -225,10 +276,10 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     double dxv = 0.0, dxp = 0.0, dyv = 0.0, dyp = 0.0;
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-      dxv += T.fcx[6 - q] * sxv[ty][tx + q];
-      dxp += T.fcx[6 - q] * sxp[ty][tx + q];
-      dyv += T.fcy[6 - q] * syv[ty + q][tx];
-      dyp += T.fcy[6 - q] * syp[ty + q][tx];
+      dxv += T.fcx[6 - q] * sxv[bf][ty][tx + q];
+      dxp += T.fcx[6 - q] * sxp[bf][ty][tx + q];
+      dyv += T.fcy[6 - q] * syv[bf][ty + q][tx];
+      dyp += T.fcy[6 - q] * syp[bf][ty + q][tx];
     }
     // Transposed radial stencil: every row kk whose DDR stencil reads plane k.
     double drv = 0.0, drp = 0.0;
@@ -244,34 +295,34 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
         }
       }
     }
-    if (k <= 6) {
-#pragma unroll
-      for (int kk = 0; kk < 3; ++kk)
-        if (kk < k - 3) {
-          const double wt = T.rad[kk].w[k];
-          drv += wt * rv_lo[kk];
-          drp += wt * rp_lo[kk];
-        }
+    // the closure rows 0..2 and nr-3..nr-1 reach planes 0..6 and nr-7..nr-1
+    // (derivative.f90:704-716,742-754): those beyond the window, on demand
+    if (k >= 4 && k <= 6) {
+      for (int kk = 0; kk < k - 3; ++kk) {
+        double uv, up;
+        radial_u(kk, uv, up);
+        const double wt = T.rad[kk].w[k];
+        drv += wt * uv;
+        drp += wt * up;
+      }
     }
-    if (k >= nr - 7) {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int kk = nr - 3 + q;
-        if (kk > k + 3) {
-          const double wt = T.rad[kk].w[k - (nr - 7)];
-          drv += wt * rv_hi[q];
-          drp += wt * rp_hi[q];
-        }
+    if (k >= nr - 7 && k <= nr - 5) {
+      for (int kk = k + 4; kk < nr; ++kk) {
+        double uv, up;
+        radial_u(kk, uv, up);
+        const double wt = T.rad[kk].w[k - (nr - 7)];
+        drv += wt * uv;
+        drp += wt * up;
       }
     }
 
     const double sg = row.sigma;
     const double tdx = T.tdx[ij], tde = T.tde[ij];
     double lt[5];
-    lt[0] = -(row.g0 / row.rho0) * u.b4 - sg * w1;
+    lt[0] = -(row.g0 * row.inv_rho0) * u.b4 - sg * w1;
     lt[1] = -row.inv_r * tdx * u.divb + dxv;
     lt[2] = -row.inv_r * tde * u.divb + dyv;
-    lt[3] = -row.drho0dr * u.b1 + row.rho0g0 * u.b5 + (2.0 / row.r) * u.divb - sg * w4 + drv;
+    lt[3] = -row.drho0dr * u.b1 + row.rho0g0 * u.b5 + (2.0 * row.inv_r) * u.divb - sg * w4 + drv;
     lt[4] = -sg * w5 + dxp + dyp + drp;
 
     const size_t x = kp + ij;
@@ -280,31 +331,28 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
 #pragma unroll
       for (int f = 0; f < 5; ++f) yo[(size_t)f * nf + x] = lt[f];
     } else {
-      const double* yi = Yin + (size_t)(P * 5) * nf;
       double* wo = Wout + (size_t)(P * 5) * nf;
       const double wv[5] = {w1, w2, w3, w4, w5};
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
-        const double yn = yi[(size_t)f * nf + x] + dt * lt[f];
-        yo[(size_t)f * nf + x] = yn;
-        if (mode & kAdjWriteW) wo[(size_t)f * nf + x] = alpha * wv[f] + beta * yn;
+        const double yn = cur.y[f] + dt * lt[f];
+        __builtin_nontemporal_store(yn, yo + (size_t)f * nf + x);
+        if (mode & kAdjWriteW) __builtin_nontemporal_store(alpha * wv[f] + beta * yn, wo + (size_t)f * nf + x);
       }
-      if (mode & kAdjGrad) {
+      if (grad) {
         // r5 = -c^2 rho0 div + ...  ->  d r5 / d c^2 = -rho0 div (zero on the BC planes)
-        const size_t g = (size_t)P * nf + x;
-        gc2[g] += dt * (-row.rho0 * divs[g]) * u.b5;
+        gc2[(size_t)P * nf + x] = cur.g + dt * (-row.rho0 * cur.dv) * u.b5;
       }
     }
 
     // Ghost adjoints of edge tiles: the receiver's DDX/DDY read ghost layer
     // l at i = -l (W), n-1+l (E), j = -l (S), n-1+l (N) with tap 3-l-i, etc.
-    const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
     if (bi == 0 && tx < 3) {
       const int l = tx + 1;
       double gp = 0.0, gv = 0.0;
       for (int ii = 0; ii <= 3 - l; ++ii) {
-        gp += T.fcx[3 - l - ii] * sxp[ty][3 + ii];
-        gv += T.fcx[3 - l - ii] * sxv[ty][3 + ii];
+        gp += T.fcx[3 - l - ii] * sxp[bf][ty][3 + ii];
+        gv += T.fcx[3 - l - ii] * sxv[bf][ty][3 + ii];
       }
       const size_t o = (size_t)(P * 2) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
       gxa_p[o] = sig_dt * gp;
@@ -314,8 +362,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = tx - (TI - 3) + 1;
       double gp = 0.0, gv = 0.0;
       for (int ii = n - 4 + l; ii <= n - 1; ++ii) {
-        gp += T.fcx[n + 2 + l - ii] * sxp[ty][3 + ii - i0];
-        gv += T.fcx[n + 2 + l - ii] * sxv[ty][3 + ii - i0];
+        gp += T.fcx[n + 2 + l - ii] * sxp[bf][ty][3 + ii - i0];
+        gv += T.fcx[n + 2 + l - ii] * sxv[bf][ty][3 + ii - i0];
       }
       const size_t o = (size_t)(P * 2 + 1) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
       gxa_p[o] = sig_dt * gp;
@@ -325,8 +373,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = ty + 1;
       double gp = 0.0, gv = 0.0;
       for (int jj = 0; jj <= 3 - l; ++jj) {
-        gp += T.fcy[3 - l - jj] * syp[3 + jj][tx];
-        gv += T.fcy[3 - l - jj] * syv[3 + jj][tx];
+        gp += T.fcy[3 - l - jj] * syp[bf][3 + jj][tx];
+        gv += T.fcy[3 - l - jj] * syv[bf][3 + jj][tx];
       }
       const size_t o = (size_t)(P * 2) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
       gya_p[o] = sig_dt * gp;
@@ -336,35 +384,72 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = ty - (TJ - 3) + 1;
       double gp = 0.0, gv = 0.0;
       for (int jj = n - 4 + l; jj <= n - 1; ++jj) {
-        gp += T.fcy[n + 2 + l - jj] * syp[3 + jj - j0][tx];
-        gv += T.fcy[n + 2 + l - jj] * syv[3 + jj - j0][tx];
+        gp += T.fcy[n + 2 + l - jj] * syp[bf][3 + jj - j0][tx];
+        gv += T.fcy[n + 2 + l - jj] * syv[bf][3 + jj - j0][tx];
       }
       const size_t o = (size_t)(P * 2 + 1) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
       gya_p[o] = sig_dt * gp;
       gya_v[o] = sig_dt * gv;
     }
-    __syncthreads();
   }
 }
 
-// Transpose of k_gather: one block per (receiver panel P, edge e, plane k).
-// Adds the ghost adjoints to the sender's rows of Y (and beta times them to
-// Wn when given).  Scalars: field sfield (+blockIdx.y); vectors: fields 1, 2.
+// Transpose of k_gather, as a gather over the sender's strips: one block per
+// (sender panel s, sender edge se, plane k).  The receiver's ghost values of
+// that strip (receiver from the inverse route) are first staged in LDS in
+// the sender's along-edge order -- for vectors already through the
+// receiver-basis transform, as theta and phi parts -- then each strip point
+// (layer l, along index a) sums the receiver stencils that read it (the
+// transposed rows gt_*), applies the sender's (xi,eta)->(theta,phi)
+// conversion transposed for vectors, and adds the result to Y (and beta
+// times it to Wn).  pass 0 covers the S/N strips, pass 1 the W/E strips:
+// strips of one pass never share a point, so the adds need no atomics;
+// corner points get their two strips in turn.
 template <bool VEC, bool RADF>
-__global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* __restrict__ gxs,
-                                                     const double* __restrict__ gys, const double* __restrict__ gxv,
-                                                     const double* __restrict__ gyv, double* Y, double* Wn,
-                                                     double beta, int sfield, long long field_stride_in) {
+__global__ __launch_bounds__(1024) void k_adj_gathert(DevTables T, const double* __restrict__ gxs,
+                                                      const double* __restrict__ gys, const double* __restrict__ gxv,
+                                                      const double* __restrict__ gyv, double* __restrict__ Y,
+                                                      double* __restrict__ Wn, double beta, int sfield,
+                                                      long long field_stride_in, int pass) {
+  extern __shared__ double sg[];  // [3 layers][n]: scalar; then theta, phi parts for vectors
   const int n = T.n, nr = T.nr;
-  const int edge = blockIdx.x;
-  const int k = edge / 24;
-  const int P = (edge % 24) / 4;
-  const int e = edge % 4;
+  const int b = blockIdx.x;
+  const int k = b / 12;
+  const int s = (b % 12) / 2;
+  const int se = (pass == 0 ? kS : kW) + (b & 1);
   const int fld = blockIdx.y;
-  const Route rt = T.route[P * 4 + e];
-  const int s = rt.sender, se = rt.sender_edge;
+  const int rv = T.inv_route[s * 4 + se];
+  const int P = (rv & 0xff) >> 2, e = rv & 3, rev = rv >> 8;
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t fo = (size_t)fld * field_stride_in;
+  const size_t kst = (size_t)3 * n;  // plane stride of both ghost layouts
+  // phase 1: ghosts (P, e, k, m, l) in the receiver layout, read in memory order
+  const double* gsrc = (e < 2) ? gxs : gys;
+  const double* vsrc = (e < 2) ? gxv : gyv;
+  const size_t gbase = (e < 2) ? fo + (size_t)(P * 2 + e) * nr * kst : fo + (size_t)(P * 2 + (e - 2)) * nr * kst;
+  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+    // x ghosts [m][l] (l fastest), y ghosts [l][m]
+    const int l = (e < 2) ? t % 3 + 1 : t / n + 1;
+    const int m = (e < 2) ? t / 3 : t % n;
+    const int mp = rev ? n - 1 - m : m;
+    double gs;
+    if (RADF) {  // FILTER_R^T over the per-plane ghost adjoints (filter.f90:682-693)
+      gs = 0.0;
+      for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
+        gs += T.filtr[k - kk + 5] * gsrc[gbase + (size_t)kk * kst + t];
+    } else {
+      gs = gsrc[gbase + (size_t)k * kst + t];
+    }
+    sg[(l - 1) * n + mp] = gs;
+    if (VEC) {
+      const double gv = vsrc[gbase + (size_t)k * kst + t];
+      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
+      sg[3 * n + (l - 1) * n + mp] = ab[0] * gv;
+      sg[6 * n + (l - 1) * n + mp] = ab[1] * gv;
+    }
+  }
+  __syncthreads();
+  // phase 2: strip points, in the sender's memory order
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
   double* ys = Y + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
@@ -373,53 +458,34 @@ __global__ __launch_bounds__(256) void k_adj_scatter(DevTables T, const double* 
   double* y2 = y1 + nf;
   double* w1 = Wn ? Wn + ((size_t)(s * 5 + 1) * nr + k) * plane : nullptr;
   double* w2 = Wn ? w1 + nf : nullptr;
+  const bool col = (se == kW || se == kE);  // column strips: layer index fastest in memory
   for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    const int l = t / n + 1, m = t % n;
-    size_t o;
-    if (e < 2)
-      o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
-    else
-      o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
-    double gs;
-    if (RADF) {
-      // FILTER_R^T of the per-plane ghost adjoints: planes kk in [5, nr-6]
-      // within 5 of k, weight filt_r[k - kk + 5] (filter.f90:682-693)
-      const double* gsrc = (e < 2) ? gxs : gys;
-      const size_t kst = (e < 2) ? (size_t)n * 3 : (size_t)3 * n;
-      const size_t o0 = o - (size_t)k * kst;
-      gs = 0.0;
-      for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
-        gs += T.filtr[k - kk + 5] * gsrc[o0 + (size_t)kk * kst];
-    } else {
-      gs = (e < 2) ? gxs[o] : gys[o];
-    }
-    double th = 0.0, ph = 0.0;
-    const int mp = rt.reverse ? n - 1 - m : m;
-    if (VEC) {
-      const double gv = (e < 2) ? gxv[o] : gyv[o];
-      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
-      th = ab[0] * gv;
-      ph = ab[1] * gv;
-    }
-    const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
-    for (int tp = 0; tp < 9; ++tp) {
-      const double wt = w[tp];
-      if (wt == 0.0) continue;
-      const int a = clampi_a(mp - 4 + tp, 0, n - 1);
-      const int idx = strip_index_a(se, l, a, n);
-      atomicAdd(ys + idx, wt * gs);
-      if (ws) atomicAdd(ws + idx, beta * wt * gs);
+    const int l = col ? t % 3 + 1 : t / n + 1;
+    const int a = col ? t / 3 : t % n;
+    const size_t r0 = ((size_t)(l - 1) * n + a) * kGtWidth;
+    double as = 0.0, ath = 0.0, aph = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGtWidth; ++q) {
+      const int mp = T.gt_mp[r0 + q];
+      const double w = T.gt_w[r0 + q];
+      as += w * sg[(l - 1) * n + mp];
       if (VEC) {
-        const double* c = cv + (size_t)idx * 6;
-        const double tb = wt * th, pb = wt * ph;
-        const double ax = c[0] * c[1] * tb + c[3] * c[4] * pb;
-        const double ay = c[0] * c[2] * tb + c[3] * c[5] * pb;
-        atomicAdd(y1 + idx, ax);
-        atomicAdd(y2 + idx, ay);
-        if (w1) {
-          atomicAdd(w1 + idx, beta * ax);
-          atomicAdd(w2 + idx, beta * ay);
-        }
+        ath += w * sg[3 * n + (l - 1) * n + mp];
+        aph += w * sg[6 * n + (l - 1) * n + mp];
+      }
+    }
+    const int idx = strip_index_a(se, l, a, n);
+    ys[idx] += as;
+    if (ws) ws[idx] += beta * as;
+    if (VEC) {
+      const double* c = cv + (size_t)idx * 6;
+      const double ax = c[0] * c[1] * ath + c[3] * c[4] * aph;
+      const double ay = c[0] * c[2] * ath + c[3] * c[5] * aph;
+      y1[idx] += ax;
+      y2[idx] += ay;
+      if (w1) {
+        w1[idx] += beta * ax;
+        w2[idx] += beta * ay;
       }
     }
   }
@@ -444,7 +510,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
   constexpr int NT = TI * TJ;
   constexpr int NCELL = (TJ + 6) * (TI + 6);
   constexpr int NC = (NCELL + NT - 1) / NT;
-  __shared__ double sb[TJ + 6][TI + 6];
+  __shared__ double sb[2][TJ + 6][TI + 6];  // alternating by plane: two barriers per plane
   __shared__ double sa[TJ][TI + 6];
   const int n = T.n, nr = T.nr;
   const int nti = n / TI, ntj = n / TJ;
@@ -507,23 +573,28 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
   // the march continues to k1+4 so that output plane k1-1 is formed.
   const int kfirst = max(k0 - kNstr, kNstr), klast = min(k1 + kNstr - 1, nr - 1 - kNstr);
   const int kstop = k1 + kNstr - 1;
-  double nx[NC];
-  if (kfirst <= klast) load_plane(kfirst, nx);
+  // two planes in flight: nx1 holds plane kk+1, nx2 plane kk+2
+  double nx1[NC], nx2[NC];
+  if (kfirst <= klast) load_plane(kfirst, nx1);
+  if (kfirst + 1 <= klast) load_plane(kfirst + 1, nx2);
   for (int kk = kfirst; kk <= kstop; ++kk) {
     double v = 0.0;
     if (kk <= klast) {  // block-uniform
+      const int pb = (kk - kfirst) & 1;
+      double (&sbp)[TJ + 6][TI + 6] = sb[pb];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const int e = tid + c * NT;
-        if (e < NCELL) (&sb[0][0])[e] = nx[c];
+        if (e < NCELL) (&sbp[0][0])[e] = nx1[c];
+        nx1[c] = nx2[c];
       }
-      if (kk + 1 <= klast) load_plane(kk + 1, nx);
+      if (kk + 2 <= klast) load_plane(kk + 2, nx2);
       __syncthreads();
       // FILTER_ETA^T: abar(row, c) = sum_q f[6-q] bbar(row-3+q, c), in-panel rows
       for (int cc = tx; cc < TI + 6; cc += TI) {
         double a = 0.0;
 #pragma unroll
-        for (int q = 0; q < 7; ++q) a += T.filt[6 - q] * sb[ty + q][cc];
+        for (int q = 0; q < 7; ++q) a += T.filt[6 - q] * sbp[ty + q][cc];
         sa[ty][cc] = a;
       }
       __syncthreads();
@@ -534,9 +605,9 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
         if (gS || gN) {  // eta ghost l sits at row -l (S) or n-1+l (N)
           double s = 0.0;
           if (gS) {
-            for (int jj = 0; jj <= 3 - ly; ++jj) s += T.filt[3 - ly - jj] * sb[3 + jj][tx + 3];
+            for (int jj = 0; jj <= 3 - ly; ++jj) s += T.filt[3 - ly - jj] * sbp[3 + jj][tx + 3];
           } else {
-            for (int jj = n - 4 + ly; jj <= n - 1; ++jj) s += T.filt[n + 2 + ly - jj] * sb[3 + jj - j0][tx + 3];
+            for (int jj = n - 4 + ly; jj <= n - 1; ++jj) s += T.filt[n + 2 + ly - jj] * sbp[3 + jj - j0][tx + 3];
           }
           gyo[(size_t)kk * 3 * n] = s;
         }
@@ -550,7 +621,6 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
           gxo[(size_t)kk * n * 3] = s;
         }
       }
-      __syncthreads();
     }
 #pragma unroll
     for (int q = 0; q < 10; ++q) win[q] = win[q + 1];
@@ -561,7 +631,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < 11; ++q) s += T.filtr[10 - q] * win[q];
-      o[(size_t)kout * plane + ij] = s;
+      __builtin_nontemporal_store(s, o + (size_t)kout * plane + ij);
     }
   }
   // kfirst - 5 <= k0 and kstop - 5 = k1 - 1: every plane of [k0, k1) was written
@@ -606,17 +676,23 @@ hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, c
                               const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
                               hipStream_t st, bool radial_filter) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
-  dim3 grid((unsigned)(24 * T.nr), (unsigned)nfields);
-  if (vec)
-    hipLaunchKernelGGL((k_adj_scatter<true, false>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                       sfield, field_stride);
-  else if (radial_filter)
-    hipLaunchKernelGGL((k_adj_scatter<false, true>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                       sfield, field_stride);
-  else
-    hipLaunchKernelGGL((k_adj_scatter<false, false>), grid, dim3(256), 0, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                       sfield, field_stride);
-  return hipGetLastError();
+  dim3 grid((unsigned)(12 * T.nr), (unsigned)nfields);
+  const int nt = std::min(1024, (3 * T.n + 63) / 64 * 64);
+  const size_t lds = sizeof(double) * (vec ? 9 : 3) * (size_t)T.n;
+  for (int pass = 0; pass < 2; ++pass) {
+    if (vec)
+      hipLaunchKernelGGL((k_adj_gathert<true, false>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                         sfield, field_stride, pass);
+    else if (radial_filter)
+      hipLaunchKernelGGL((k_adj_gathert<false, true>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                         sfield, field_stride, pass);
+    else
+      hipLaunchKernelGGL((k_adj_gathert<false, false>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                         sfield, field_stride, pass);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <int TI, int TJ>

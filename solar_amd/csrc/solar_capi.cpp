@@ -188,6 +188,11 @@ int solar::upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allo
   for (int p = 0; p < 6; ++p)
     for (int e = 0; e < 4; ++e) routes[p * 4 + e] = t.route[p][e];
   if ((rc = put(&d->route, routes))) return rc;
+  if ((rc = put(&d->gt_mp, t.gt_mp)) || (rc = put(&d->gt_w, t.gt_w)) ||
+      (rc = put(&d->adj_metric, t.adj_metric)))
+    return rc;
+  for (int p = 0; p < 6; ++p)
+    for (int e = 0; e < 4; ++e) d->inv_route[p * 4 + e] = t.inv_route[p][e];
   for (int q = 0; q < 7; ++q) {
     d->fcx[q] = t.fcx[q];
     d->fcy[q] = t.fcy[q];

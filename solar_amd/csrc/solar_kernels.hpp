@@ -20,6 +20,10 @@ struct DevTables {
   const double* gw;      // gather weights         [3][n][9]
   const double* trans;   // receiver transform     [6][4][3][n][2]
   const Route* route;    // panel routes           [6][4]
+  const int* gt_mp;      // transposed gather rows [3][n][kGtWidth]: receiver along index
+  const double* gt_w;    //   and weight per entry
+  const double* adj_metric;  // adjoint metric factors [n*n][4]
+  int inv_route[24];     // (receiver P*4+e) | reverse << 8 per (sender, sender edge)
   double fcx[7], fcy[7];
   double filt[7], filtr[11];
 };

@@ -3,6 +3,7 @@
 // reference's, so the tables match its in-loop values bit for bit.
 #include "solar_tables.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace solar {
@@ -309,7 +310,7 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
     row.drho0dr = s;
     row.sigma = sponge_sigma(k + 1, nr, sponge_len, a_sponge);
     for (int q = 0; q < 7; ++q) row.w[q] = wk[static_cast<size_t>(k) * 7 + q];
-    row.pad = 0.0;
+    row.inv_rho0 = 1.0 / row.rho0;
   }
   for (int q = 0; q < 7; ++q) {
     t->fcx[q] = (1.0 / t->dxi) * kCoeffs1[q];
@@ -317,6 +318,51 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
   }
   for (int p = 0; p < 6; ++p)
     for (int e = 0; e < 4; ++e) t->route[p][e] = kRoutes[p][e];
+  for (int p = 0; p < 6; ++p)
+    for (int e = 0; e < 4; ++e) {
+      const Route& r = kRoutes[p][e];
+      t->inv_route[r.sender][r.sender_edge] = (p * 4 + e) | (r.reverse << 8);
+    }
+  t->adj_metric.resize(static_cast<size_t>(n) * n * 4);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const size_t ij = static_cast<size_t>(j) * n + i;
+      t->adj_metric[ij * 4 + 0] = t->dc[ij] / t->metric_c[i];
+      t->adj_metric[ij * 4 + 1] = t->dc[ij] / t->metric_d[j];
+      t->adj_metric[ij * 4 + 2] = t->xy[ij] / t->metric_c[i];
+      t->adj_metric[ij * 4 + 3] = t->xy[ij] / t->metric_d[j];
+    }
+  // Transposed gather rows (taps clamped to the edge like the gather; equal
+  // (a, mp) pairs merged; unused entries keep weight 0 at mp = a).
+  t->gt_mp.assign(static_cast<size_t>(3) * n * kGtWidth, 0);
+  t->gt_w.assign(static_cast<size_t>(3) * n * kGtWidth, 0.0);
+  for (int l = 1; l <= 3; ++l)
+    for (int a = 0; a < n; ++a) {
+      const size_t row = (static_cast<size_t>(l - 1) * n + a) * kGtWidth;
+      int cnt = 0;
+      for (int q = 0; q < kGtWidth; ++q) t->gt_mp[row + q] = a;
+      for (int mp = std::max(0, a - 8); mp <= std::min(n - 1, a + 8); ++mp) {
+        double w = 0.0;
+        bool any = false;
+        for (int tp = 0; tp < 9; ++tp) {
+          const int src = std::min(std::max(mp - 4 + tp, 0), n - 1);
+          const double wt = t->gather_w[(static_cast<size_t>(l - 1) * n + mp) * 9 + tp];
+          if (src == a && wt != 0.0) {
+            w += wt;
+            any = true;
+          }
+        }
+        if (any) {
+          if (cnt == kGtWidth) {
+            if (err) *err = "transposed gather row wider than kGtWidth";
+            return false;
+          }
+          t->gt_mp[row + cnt] = mp;
+          t->gt_w[row + cnt] = w;
+          ++cnt;
+        }
+      }
+    }
   return true;
 }
 

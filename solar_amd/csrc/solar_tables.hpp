@@ -39,7 +39,7 @@ struct RadialRow {
   double drho0dr;  // DDR(rho0)                    :51
   double sigma;    // sponge coefficient           sponge.f90:235-241
   double w[7];     // DDR weights on planes base..base+6 (derivative.f90:656-754)
-  double pad;
+  double inv_rho0;  // 1/rho0(k) (used by the adjoint only)
 };
 static_assert(sizeof(RadialRow) == 16 * sizeof(double), "RadialRow layout");
 
@@ -61,6 +61,15 @@ struct Tables {
   // Ghost gather weights: same shape as interpolant, with the corner copies
   // (interpolation.f90:183-184,304-305) written as a single unit weight.
   std::vector<double> gather_w;
+  // Transpose of the gather along an edge: for layer l and sender along
+  // index a, the receiver stencils (mp = receiver along index before any
+  // reversal, weight) that read a, as fixed-width rows of kGtWidth entries
+  // ((l-1)*n + a)*kGtWidth + q, padded with zero weights.
+  std::vector<int> gt_mp;
+  std::vector<double> gt_w;
+  // Adjoint metric factors per point, interleaved (j*n + i)*4 + c:
+  // metric_delta/CD / C, metric_delta/CD / D, XY / C, XY / D.
+  std::vector<double> adj_metric;
   // Ghost coordinates of transform.f90:142-152, index l = 1..3.
   double xm[4]{}, xp[4]{}, ym[4]{}, yp[4]{}, cm[4]{}, cp[4]{}, dm[4]{}, dp[4]{};
   // Basis conversion (xi,eta) -> (theta,phi) per panel type (0: panels 1-4,
@@ -75,7 +84,11 @@ struct Tables {
   std::vector<RadialRow> radial;
   double fcx[7]{}, fcy[7]{};  // (1/dxi)*coeffs1, (1/deta)*coeffs1
   Route route[6][4]{};
+  // For sender panel s and sender edge se: (receiver P*4 + e) | reverse << 8.
+  int inv_route[6][4]{};
 };
+
+constexpr int kGtWidth = 12;
 
 // Scheme constants of header:40-133.
 extern const double kCoeffs1[7];
