@@ -3,11 +3,14 @@
 ! driver (cube.f90:35-92): background (SOLAR_DATA), set-up of metric and
 ! interpolation tables, time step, initial condition, then the time loop of
 ! RK4 calls with the per-step print of step, dt, time, the v_r norm and the
-! step wall time.  The numerics below RK4 run on the GPU through SOLAR_CAPI.
+! step wall time, and every 6th step the visual and radial-trace output.
+! The numerics below RK4 run on the GPU through SOLAR_CAPI.
 !
-! usage: cube_gpu n nr background ic nsteps seed [state_out]
+! usage: cube_gpu n nr background ic nsteps seed [state_out [outdir]]
 !   background 0 uniform / 1 stratified, ic 0 Gaussian pulse / 1 Y_lm mode
 !   state_out: file that receives the final q(n,n,nr,5,6) as raw float64
+!   outdir: every 6th step WRITE_VISUAL and WRITE_RADIAL_TRACE there, as
+!           cube.f90:84-89 (obsheight 0.98); '-' or absent: no output
 !-----------------------------------------------------------------------
 PROGRAM CUBE_GPU
 
@@ -15,14 +18,15 @@ PROGRAM CUBE_GPU
  use SOLAR_CAPI
  implicit none
 
- integer(c_int) :: n, nr, bg_kind, ic_kind, nsteps, sponge_len, rc
+ integer(c_int) :: n, nr, bg_kind, ic_kind, nsteps, sponge_len, rc, o_rad, cue_vis
  integer(c_int64_t) :: seed
  real(c_double), allocatable :: gr(:), c2(:), rho0(:), g0(:), q(:)
- real(c_double) :: params(3), info(4), dt, time, dxi, rms, a_sponge
+ real(c_double) :: params(3), info(4), dt, time, dxi, rms, tt, a_sponge
  type(c_ptr) :: ctx
  integer :: step, nargs, u
  integer(8) :: t0, t1, rate
- character(len=256) :: arg, state_out
+ character(len=256) :: arg, state_out, outdir
+ character(kind=c_char, len=257) :: cdir
 
  nargs = command_argument_count()
  if (nargs < 6) then
@@ -37,6 +41,11 @@ PROGRAM CUBE_GPU
  call get_command_argument(6, arg); read(arg, *) seed
  state_out = ' '
  if (nargs >= 7) call get_command_argument(7, state_out)
+ if (trim(state_out) == '-') state_out = ' '
+ outdir = ' '
+ if (nargs >= 8) call get_command_argument(8, outdir)
+ if (trim(outdir) == '-') outdir = ' '
+ cdir = trim(outdir) // c_null_char
 
  ! SOLAR_DATA (cube.f90:37): nondimensional radial background
  allocate(gr(nr), c2(nr), rho0(nr), g0(nr))
@@ -65,6 +74,10 @@ PROGRAM CUBE_GPU
  rc = solar_set_state(ctx, q)
  call SOLAR_CHECK(rc, 'solar_set_state')
  print *, 'nstep = ', nsteps
+ ! output height (ic_acoustic_wave.f90:128-133)
+ rc = solar_output_radius(nr, gr, 0.98d0, o_rad)
+ call SOLAR_CHECK(rc, 'solar_output_radius')
+ cue_vis = 0
 
  time = 0d0
  call system_clock(count_rate=rate)
@@ -75,10 +88,19 @@ PROGRAM CUBE_GPU
   call system_clock(t1)
   rc = solar_norm_vr(ctx, rms)
   call SOLAR_CHECK(rc, 'solar_norm_vr')
+  rc = solar_norm(ctx, 3, 1, tt)                 ! tt = norm(q(:,:,:,4)), cube.f90:77
+  call SOLAR_CHECK(rc, 'solar_norm')
   if (mod(step, max(1, nsteps / 10)) == 0 .or. step == nsteps - 1) then
    print *, step, dt, time
-   print *, rms
+   print *, tt, rms
    print *, 'steptime = ', dble(t1 - t0) / dble(rate)
+  endif
+  if (len_trim(outdir) > 0 .and. mod(step, 6) == 0) then   ! cube.f90:84-89
+   rc = solar_write_visual(ctx, cdir, cue_vis, o_rad)
+   call SOLAR_CHECK(rc, 'solar_write_visual')
+   rc = solar_write_radial_trace(ctx, cdir)
+   call SOLAR_CHECK(rc, 'solar_write_radial_trace')
+   cue_vis = cue_vis + 1
   endif
  enddo
 

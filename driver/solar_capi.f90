@@ -90,6 +90,87 @@ MODULE SOLAR_CAPI
    real(c_double), intent(out) :: q(*), info(*)
   end function
 
+  ! cube.f90:77 diagnostic: reference_formula /= 0 gives auxiliary.f90:100-125 as written
+  integer(c_int) function solar_norm(ctx, field, reference_formula, out) bind(C, name='solar_norm')
+   import :: c_int, c_ptr, c_double
+   type(c_ptr), value :: ctx
+   integer(c_int), value :: field, reference_formula
+   real(c_double), intent(out) :: out
+  end function
+
+  ! WRITE_VISUAL / WRITE_RADIAL_TRACE / WRITE_RESTART (output.f90) of the device state;
+  ! dir is a NUL-terminated path (c_null_char alone: the working directory)
+  integer(c_int) function solar_write_visual(ctx, dir, cue, o_rad) bind(C, name='solar_write_visual')
+   import :: c_int, c_ptr, c_char
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+   integer(c_int), value :: cue, o_rad
+  end function
+
+  integer(c_int) function solar_write_radial_trace(ctx, dir) bind(C, name='solar_write_radial_trace')
+   import :: c_int, c_ptr, c_char
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+  end function
+
+  integer(c_int) function solar_write_restart(ctx, dir, cue, time, step) bind(C, name='solar_write_restart')
+   import :: c_int, c_ptr, c_char, c_double
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+   integer(c_int), value :: cue, step
+   real(c_double), value :: time
+  end function
+
+  integer(c_int) function solar_read_restart(ctx, dir, cue, time, step) bind(C, name='solar_read_restart')
+   import :: c_int, c_ptr, c_char, c_double
+   type(c_ptr), value :: ctx
+   character(kind=c_char), intent(in) :: dir(*)
+   integer(c_int), value :: cue
+   real(c_double), intent(out) :: time
+   integer(c_int), intent(out) :: step
+  end function
+
+  integer(c_int) function solar_output_radius(nr, gr, obsheight, o_rad) bind(C, name='solar_output_radius')
+   import :: c_int, c_double
+   integer(c_int), value :: nr
+   real(c_double), intent(in) :: gr(*)
+   real(c_double), value :: obsheight
+   integer(c_int), intent(out) :: o_rad
+  end function
+
+  ! Sensitivities: 3-D c^2 field, receivers, traces, misfit gradient
+  integer(c_int) function solar_set_c2_field(ctx, c2) bind(C, name='solar_set_c2_field')
+   import :: c_int, c_ptr, c_double
+   type(c_ptr), value :: ctx
+   real(c_double), intent(in) :: c2(*)
+  end function
+
+  integer(c_int) function solar_set_receivers(ctx, nrec, pkji) bind(C, name='solar_set_receivers')
+   import :: c_int, c_ptr
+   type(c_ptr), value :: ctx
+   integer(c_int), value :: nrec
+   integer(c_int), intent(in) :: pkji(*)
+  end function
+
+  integer(c_int) function solar_record(ctx, nsteps, dt, seis) bind(C, name='solar_record')
+   import :: c_int, c_ptr, c_double
+   type(c_ptr), value :: ctx
+   integer(c_int), value :: nsteps
+   real(c_double), value :: dt
+   real(c_double), intent(out) :: seis(*)
+  end function
+
+  integer(c_int) function solar_misfit_gradient(ctx, q0, nsteps, dt, data, checkpoint_every, J, grad, seis, &
+                                                segment_used) bind(C, name='solar_misfit_gradient')
+   import :: c_int, c_ptr, c_double
+   type(c_ptr), value :: ctx
+   real(c_double), intent(in) :: q0(*), data(*)
+   integer(c_int), value :: nsteps, checkpoint_every
+   real(c_double), value :: dt
+   real(c_double), intent(out) :: J, grad(*), seis(*)
+   integer(c_int), intent(out) :: segment_used
+  end function
+
   type(c_ptr) function solar_last_error() bind(C, name='solar_last_error')
    import :: c_ptr
   end function
