@@ -139,6 +139,13 @@ int solar_device_bytes(solar_ctx* ctx, size_t* bytes);
  * The host array is copied. */
 int solar_set_c2_field(solar_ctx* ctx, const double* c2);
 
+/* Storage of the stage divergences the misfit gradient keeps between its
+ * forward and reverse passes: div_bits = 64 (default; the gradient is the
+ * exact discrete gradient to rounding) or 32 (half the memory per stored
+ * step, so longer segments and less recompute; the gradient then carries the
+ * FP32 rounding of div(v), about 1e-7 relative). */
+int solar_set_adjoint_options(solar_ctx* ctx, int div_bits);
+
 /* Receivers: nrec points of the p' field given as (panel, k, j, i) 0-based
  * quadruples in pkji[4*nrec]. */
 int solar_set_receivers(solar_ctx* ctx, int nrec, const int* pkji);
@@ -251,6 +258,14 @@ int solar_ens_sync(solar_ens* ens);
 
 /* Grid points over all members, launches per step, device bytes held. */
 int solar_ens_info(solar_ens* ens, size_t* points, int* launches_per_step, size_t* device_bytes);
+
+/* Phases of the last solar_misfit_gradient, seconds: out[0] workspace
+ * allocation (0 when the handle's workspace from an earlier call fits),
+ * out[1] forward pass (host clock), out[2] segment recomputes and out[3]
+ * adjoint steps (HIP events on the handle's stream).  The workspace
+ * (checkpoints, divergences, traces) stays with the handle until
+ * solar_finalize. */
+int solar_adjoint_times(solar_ctx* ctx, double* out);
 
 /* Description of the last error on this thread ("" if none). */
 const char* solar_last_error(void);

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
                                                        double* __restrict__ gxa_p, double* __restrict__ gya_p,
                                                        double* __restrict__ gxa_v, double* __restrict__ gya_v,
                                                        double dt, double alpha, double beta, double wscale,
-                                                       int kchunk, int nkch, int mode) {
+                                                       int kchunk, int nkch, int mode, int div32) {
   // Transposed-stencil inputs with i halos (x) and j halos (y); two buffers
   // alternating by plane, so one barrier per plane suffices.
   __shared__ double sxv[2][TJ][TI + 6], sxp[2][TJ][TI + 6];
@@ -199,7 +199,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     }
     if (grad) {
       const size_t g = (size_t)P * nf + (size_t)k * plane + ij;
-      L.dv = __builtin_nontemporal_load(divs + g);
+      L.dv = div32 ? static_cast<double>(__builtin_nontemporal_load(reinterpret_cast<const float*>(divs) + g))
+                   : __builtin_nontemporal_load(divs + g);
       L.g = gc2[g];
     }
     if (h_in) {
@@ -500,13 +501,63 @@ __global__ __launch_bounds__(1024) void k_adj_gathert(DevTables T, const double*
 // the transposed stencils have no ghosts) goes to LDS, FILTER_ETA^T runs on
 // every column, FILTER_XI^T on the in-panel rows; the result enters an
 // 11-plane register window from which FILTER_R^T forms the output plane
-// kk - 5.  Edge tiles write the per-plane ghost adjoints of both angular
-// passes (the eta ghosts from bbar itself: FILTER_ETA reads unfiltered
-// ghosts, filter.f90:43); planes outside [5, nr-6] get zeros.
+// kk - 5.  The ghost adjoints of both angular passes come from
+// k_adj_filter_ghosts.
+// Per-plane ghost adjoints of FILTER_XI^T and FILTER_ETA^T (the receiver's
+// filters read ghost layer l at i = -l (W), n-1+l (E), j = -l (S), n-1+l (N)
+// with tap 3-l-i, etc.), for planes kk in [5, nr-6]; zero on the others
+// (FILTER_R reads no other plane's output).  The eta ghosts come from bbar
+// itself (FILTER_ETA reads unfiltered ghosts, filter.f90:43), the xi ghosts
+// from abar = FILTER_ETA^T bbar on the edge columns.  One block per (panel,
+// field, edge, plane), one thread per (layer, along index).
+__global__ __launch_bounds__(256) void k_adj_filter_ghosts(DevTables T, const double* __restrict__ bin,
+                                                           double* __restrict__ fgxa, double* __restrict__ fgya) {
+  const int n = T.n, nr = T.nr;
+  int b = blockIdx.x;
+  const int kk = b % nr;
+  b /= nr;
+  const int e = b % 4;
+  const int PF = b / 4;
+  const int P = PF / 5, F = PF % 5;
+  const size_t plane = (size_t)n * n, nf = plane * nr;
+  const double* g = bin + (size_t)PF * nf + (size_t)kk * plane;
+  const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
+  const size_t fst = (size_t)12 * gx_side;
+  const bool live = kk >= kNstr && kk <= nr - 1 - kNstr;
+  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
+    if (e >= 2) {  // S / N: rows 0..2 or n-3..n-1 of bbar, along i
+      const int l = t / n + 1, i = t % n;
+      double s = 0.0;
+      if (live) {
+        if (e == 2) {
+          for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * g[(size_t)jj * n + i];
+        } else {
+          for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * g[(size_t)jj * n + i];
+        }
+      }
+      fgya[F * fst + (size_t)(P * 2 + (e - 2)) * gy_side + ((size_t)kk * 3 + (l - 1)) * n + i] = s;
+    } else {  // W / E: abar on columns 0..2 or n-3..n-1, along j (layer index fastest)
+      const int l = t % 3 + 1, j = t / 3;
+      double s = 0.0;
+      if (live) {
+        const int ia = (e == 0) ? 0 : n - 4 + l, ib = (e == 0) ? 3 - l : n - 1;
+        for (int ii = ia; ii <= ib; ++ii) {
+          double a = 0.0;  // abar(ii, j) = sum_q f[6-q] bbar(ii, j-3+q), zero beyond the panel
+          for (int q = 0; q < 7; ++q) {
+            const int jj = j - 3 + q;
+            if (jj >= 0 && jj < n) a += T.filt[6 - q] * g[(size_t)jj * n + ii];
+          }
+          s += ((e == 0) ? T.filt[3 - l - ii] : T.filt[n + 2 + l - ii]) * a;
+        }
+      }
+      fgxa[F * fst + (size_t)(P * 2 + e) * gx_side + ((size_t)kk * n + j) * 3 + (l - 1)] = s;
+    }
+  }
+}
+
 template <int TI, int TJ>
 __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double* __restrict__ bin,
-                                                        double* __restrict__ out, double* __restrict__ fgxa,
-                                                        double* __restrict__ fgya, int kchunk, int nkch) {
+                                                        double* __restrict__ out, int kchunk, int nkch) {
   constexpr int NT = TI * TJ;
   constexpr int NCELL = (TJ + 6) * (TI + 6);
   constexpr int NC = (NCELL + NT - 1) / NT;
@@ -522,7 +573,6 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
   b /= ntj;
   const int PF = b % 30;
   const int ch = b / 30;
-  const int P = PF / 5, F = PF % 5;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
@@ -533,20 +583,6 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
   const double* g = bin + (size_t)PF * nf;
   double* o = out + (size_t)PF * nf;
   const size_t ij = (size_t)j * n + i;
-  const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
-  const size_t fst = (size_t)12 * gx_side;
-  const bool gW = bi == 0 && tx < 3, gE = bi == nti - 1 && tx >= TI - 3;
-  const bool gS = bj == 0 && ty < 3, gN = bj == ntj - 1 && ty >= TJ - 3;
-  const int lx = gW ? tx + 1 : tx - (TI - 3) + 1, ly = gS ? ty + 1 : ty - (TJ - 3) + 1;
-  double* gxo = fgxa + F * fst + (size_t)(P * 2 + (gW ? 0 : 1)) * gx_side + (size_t)j * 3 + (lx - 1);
-  double* gyo = fgya + F * fst + (size_t)(P * 2 + (gS ? 0 : 1)) * gy_side + (size_t)(ly - 1) * n + i;
-
-  // ghost adjoints of the planes of this chunk that carry none
-  for (int k = k0; k < k1; ++k)
-    if (k < kNstr || k >= nr - kNstr) {
-      if (gW || gE) gxo[(size_t)k * n * 3] = 0.0;
-      if (gS || gN) gyo[(size_t)k * 3 * n] = 0.0;
-    }
 
   // cells of the tile this thread loads
   const int tid = ty * TI + tx;
@@ -601,26 +637,6 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_filter(DevTables T, const double
       // FILTER_XI^T
 #pragma unroll
       for (int q = 0; q < 7; ++q) v += T.filt[6 - q] * sa[ty][tx + q];
-      if (kk >= k0 && kk < k1) {
-        if (gS || gN) {  // eta ghost l sits at row -l (S) or n-1+l (N)
-          double s = 0.0;
-          if (gS) {
-            for (int jj = 0; jj <= 3 - ly; ++jj) s += T.filt[3 - ly - jj] * sbp[3 + jj][tx + 3];
-          } else {
-            for (int jj = n - 4 + ly; jj <= n - 1; ++jj) s += T.filt[n + 2 + ly - jj] * sbp[3 + jj - j0][tx + 3];
-          }
-          gyo[(size_t)kk * 3 * n] = s;
-        }
-        if (gW || gE) {
-          double s = 0.0;
-          if (gW) {
-            for (int ii = 0; ii <= 3 - lx; ++ii) s += T.filt[3 - lx - ii] * sa[ty][3 + ii];
-          } else {
-            for (int ii = n - 4 + lx; ii <= n - 1; ++ii) s += T.filt[n + 2 + lx - ii] * sa[ty][3 + ii - i0];
-          }
-          gxo[(size_t)kk * n * 3] = s;
-        }
-      }
     }
 #pragma unroll
     for (int q = 0; q < 10; ++q) win[q] = win[q + 1];
@@ -663,7 +679,7 @@ static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, 
   const int kchunk = (T.nr + nkch - 1) / nkch;
   hipLaunchKernelGGL((k_adj_stage<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout, a.Wout,
                      a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, a.wscale,
-                     kchunk, nkch, a.mode);
+                     kchunk, nkch, a.mode, a.div32);
   return hipGetLastError();
 }
 
@@ -704,8 +720,8 @@ static hipError_t launch_adj_filter_t(const DevTables& T, const double* in, doub
   const int maxk = T.nr / 48 > 0 ? T.nr / 48 : 1;
   if (nkch > maxk) nkch = maxk;
   const int kchunk = (T.nr + nkch - 1) / nkch;
-  hipLaunchKernelGGL((k_adj_filter<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, in, out, fgxa, fgya, kchunk,
-                     nkch);
+  hipLaunchKernelGGL((k_adj_filter<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, in, out, kchunk, nkch);
+  hipLaunchKernelGGL(k_adj_filter_ghosts, dim3(30 * 4 * T.nr), dim3(256), 0, st, T, in, fgxa, fgya);
   return hipGetLastError();
 }
 

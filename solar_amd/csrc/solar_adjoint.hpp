@@ -21,6 +21,7 @@ struct AdjStageArgs {
   double *gxa_p = nullptr, *gya_p = nullptr, *gxa_v = nullptr, *gya_v = nullptr;
   double dt = 1.0, alpha = 0.0, beta = 0.0, wscale = 1.0;
   int mode = 0;
+  int div32 = 0;  // divs holds float values
 };
 
 hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream_t st);

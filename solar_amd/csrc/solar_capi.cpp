@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -72,6 +73,10 @@ struct solar_ctx {
   double* c2f_graph = nullptr;
   double *wb[2] = {nullptr, nullptr}, *tmp = nullptr, *lam = nullptr, *gc2 = nullptr;
   long long* rec = nullptr;  // receiver indices into the state (field p')
+  int div_bits = 64;         // storage of the stage divergences kept for the gradient
+  void* adj_ws = nullptr;    // workspace of solar_misfit_gradient (checkpoints, divergences, traces), grow-only
+  size_t adj_ws_bytes = 0;
+  double adj_times[4] = {0, 0, 0, 0};  // last gradient: workspace, forward, recompute, adjoint (s)
   int nrec = 0, rec_cap = 0;
 };
 
@@ -91,6 +96,8 @@ int dev_alloc(solar_ctx* c, T** p, size_t count) {
 }
 
 void release(solar_ctx* c) {
+  if (c->adj_ws) (void)hipFree(c->adj_ws);
+  c->adj_ws = nullptr;
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -112,7 +119,7 @@ int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = null
     HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
     HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
-                               c->shape, c->c2f, divp ? divp[s] : nullptr));
+                               c->shape, c->c2f, divp ? divp[s] : nullptr, c->div_bits == 32));
     void* t = cur;
     cur = nxt;
     nxt = t;
@@ -617,6 +624,7 @@ static int enqueue_adj_step(solar_ctx* c, double dt, double* lam, double* const*
     a.beta = s > 1 ? solar::kBetaRk4[s - 2] : 0.0;
     a.mode = (s > 1 ? solar::kAdjWriteW : 0) | (divp ? solar::kAdjGrad : 0);
     a.divs = divp ? divp[s - 1] : nullptr;
+    a.div32 = c->div_bits == 32;
     a.gc2 = c->gc2;
     a.c2f = c->c2f;
     a.gxa_p = c->gxp;
@@ -626,6 +634,12 @@ static int enqueue_adj_step(solar_ctx* c, double dt, double* lam, double* const*
     HIPCHK(solar::launch_adj_stage(T, a, c->stream));
     HIPCHK(solar::launch_adj_scatter(T, true, c->gxp, c->gyp, c->gxv, c->gyv, lam, a.Wout, a.beta, 4, 1, c->stream));
   }
+  return SOLAR_OK;
+}
+
+int solar_set_adjoint_options(solar_ctx* c, int div_bits) {
+  if (!c || (div_bits != 64 && div_bits != 32)) return fail(SOLAR_ERR_ARG, "solar_set_adjoint_options: div_bits 64 or 32");
+  c->div_bits = div_bits;
   return SOLAR_OK;
 }
 
@@ -743,22 +757,33 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
   int rc = adj_alloc(c);
   if (rc) return rc;
   const size_t nq = c->nf * 30, nf6 = c->nf * 6;
-  const size_t step_div = 5 * nf6 * sizeof(double), ck_bytes = nq * sizeof(double);
+  const size_t esz_div = c->div_bits == 32 ? sizeof(float) : sizeof(double);
+  const size_t step_div = 5 * nf6 * esz_div, ck_bytes = nq * sizeof(double);
   const size_t trace_bytes = 2 * static_cast<size_t>(nsteps) * c->nrec * sizeof(double);
   // Segment length L: checkpoint_every, or (<= 0) the longest that fits.  The
   // last L steps keep their stage divergences from the forward pass; every
   // earlier segment is recomputed once from its checkpoint.
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };  // 256-byte aligned parts of one workspace
   auto need = [&](int L) {
     const size_t nseg = (static_cast<size_t>(nsteps) + L - 1) / L;
-    return static_cast<size_t>(L) * step_div + (nseg - 1) * ck_bytes + trace_bytes;
+    return up(static_cast<size_t>(L) * step_div) + up((nseg - 1) * ck_bytes) + 2 * up(trace_bytes / 2);
   };
   size_t freeb = 0, totalb = 0;
   HIPCHK(hipMemGetInfo(&freeb, &totalb));
+  freeb += c->adj_ws_bytes;  // the handle's workspace is reused or replaced
   const size_t margin = size_t(2) << 30;
   const size_t avail = freeb > margin ? freeb - margin : 0;
   int L = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
-  if (checkpoint_every <= 0)
+  if (checkpoint_every <= 0) {
     while (L > 1 && need(L) > avail) --L;
+    // keep the handle's workspace when it holds nearly as long a segment (a
+    // new allocation of hundreds of GB costs seconds)
+    if (c->adj_ws_bytes && need(L) > c->adj_ws_bytes) {
+      int Lw = L;
+      while (Lw > 1 && need(Lw) > c->adj_ws_bytes) --Lw;
+      if (need(Lw) <= c->adj_ws_bytes && Lw * 10 >= L * 9) L = Lw;
+    }
+  }
   if (need(L) > avail)
     return fail(SOLAR_ERR_ARG, "solar_misfit_gradient: needs " + std::to_string(need(L) >> 20) + " MiB, " +
                                    std::to_string(avail >> 20) + " MiB free" +
@@ -769,23 +794,32 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
     n1[g] = nsteps - (nseg - 1 - g) * L;
     n0[g] = std::max(0, n1[g] - L);
   }
-  double *ck = nullptr, *divs = nullptr, *seis = nullptr, *dat = nullptr;
-  std::vector<void*> mine;
-  auto grab = [&](double** p, size_t bytes) -> int {
+  // one workspace, kept by the handle for the next call: checkpoints | divergences | traces | data
+  const size_t b_ck = up((nseg - 1) * ck_bytes), b_div = up(static_cast<size_t>(L) * step_div);
+  const size_t b_tr = up(trace_bytes / 2), ws_need = b_ck + b_div + 2 * b_tr;
+  const auto t_start = std::chrono::steady_clock::now();
+  if (ws_need > c->adj_ws_bytes) {
+    if (c->adj_ws) (void)hipFree(c->adj_ws);
+    c->adj_ws = nullptr;
+    c->adj_ws_bytes = 0;
     void* v = nullptr;
-    hipError_t e = hipMalloc(&v, bytes ? bytes : 8);
-    if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    mine.push_back(v);
-    *p = static_cast<double*>(v);
-    return SOLAR_OK;
-  };
-  if ((rc = grab(&ck, (nseg - 1) * ck_bytes)) || (rc = grab(&divs, static_cast<size_t>(L) * step_div)) ||
-      (rc = grab(&seis, trace_bytes / 2)) || (rc = grab(&dat, trace_bytes / 2))) {
-    for (void* v : mine) (void)hipFree(v);
-    return rc;
+    hipError_t e = hipMalloc(&v, ws_need);
+    if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("hipMalloc (gradient workspace): ") + hipGetErrorString(e));
+    c->adj_ws = v;
+    c->adj_ws_bytes = ws_need;
   }
+  char* wsb = static_cast<char*>(c->adj_ws);
+  double* ck = reinterpret_cast<double*>(wsb);
+  double* divs = reinterpret_cast<double*>(wsb + b_ck);
+  double* seis = reinterpret_cast<double*>(wsb + b_ck + b_div);
+  double* dat = reinterpret_cast<double*>(wsb + b_ck + b_div + b_tr);
+  const auto t_ws = std::chrono::steady_clock::now();
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+  float ms_rec = 0.f, ms_adj = 0.f, ms_fwd = 0.f;
   std::vector<double*> dp(static_cast<size_t>(L) * 5);
-  for (size_t x = 0; x < dp.size(); ++x) dp[x] = divs + x * nf6;
+  for (size_t x = 0; x < dp.size(); ++x)
+    dp[x] = reinterpret_cast<double*>(reinterpret_cast<char*>(divs) + x * nf6 * esz_div);
   const size_t nrec = static_cast<size_t>(c->nrec);
   auto run = [&]() -> int {
     int r;
@@ -807,7 +841,12 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
       HIPCHK(solar::launch_sample(static_cast<const double*>(c->q[0]), c->rec, c->nrec, seis + n * nrec, c->stream));
     }
     std::vector<double> sh(static_cast<size_t>(nsteps) * nrec);
+    HIPCHK(hipEventRecord(ev[0], c->stream));
     if ((r = download_f64(c, sh.data(), seis, sh.size()))) return r;
+    {
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_ws).count();
+      ms_fwd = static_cast<float>(s * 1e3);
+    }
     double jj = 0.0;
     for (size_t x = 0; x < sh.size(); ++x) jj += (sh[x] - data[x]) * (sh[x] - data[x]);
     *J = jj;
@@ -816,23 +855,42 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
     HIPCHK(hipMemsetAsync(c->lam, 0, nq * sizeof(double), c->stream));
     HIPCHK(hipMemsetAsync(c->gc2, 0, nf6 * sizeof(double), c->stream));
     for (int g = nseg - 1; g >= 0; --g) {
+      HIPCHK(hipEventRecord(ev[0], c->stream));
       if (g < nseg - 1) {  // recompute the segment from its checkpoint, keeping the divergences
         HIPCHK(hipMemcpyAsync(c->q[0], ck + static_cast<size_t>(g) * nq, ck_bytes, hipMemcpyDeviceToDevice,
                               c->stream));
         for (int m = n0[g]; m < n1[g]; ++m)
           if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - n0[g]) * 5]))) return r;
       }
+      HIPCHK(hipEventRecord(ev[1], c->stream));
       for (int n = n1[g] - 1; n >= n0[g]; --n) {
         HIPCHK(solar::launch_inject(c->lam, c->rec, c->nrec, seis + n * nrec, dat + n * nrec, c->stream));
         if ((r = enqueue_adj_step(c, dt, c->lam, &dp[static_cast<size_t>(n - n0[g]) * 5]))) return r;
       }
+      HIPCHK(hipEventRecord(ev[2], c->stream));
+      HIPCHK(hipEventSynchronize(ev[2]));
+      float a = 0.f, b = 0.f;
+      HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[1], ev[2]));
+      ms_rec += a;
+      ms_adj += b;
     }
     return download_f64(c, grad, c->gc2, nf6);
   };
   rc = run();
   (void)hipStreamSynchronize(c->stream);
-  for (void* v : mine) (void)hipFree(v);
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  c->adj_times[0] = std::chrono::duration<double>(t_ws - t_start).count();
+  c->adj_times[1] = ms_fwd * 1e-3;
+  c->adj_times[2] = ms_rec * 1e-3;
+  c->adj_times[3] = ms_adj * 1e-3;
   if (segment_used) *segment_used = L;
   return rc;
+}
+
+int solar_adjoint_times(solar_ctx* c, double* out) {
+  if (!c || !out) return fail(SOLAR_ERR_ARG, "solar_adjoint_times: null argument");
+  for (int x = 0; x < 4; ++x) out[x] = c->adj_times[x];
+  return SOLAR_OK;
 }
 }  // extern "C"

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
                                                    const double* __restrict__ gxv_a, const double* __restrict__ gyv_a,
                                                    double alpha, double beta, double dt_a, int kchunk, int nkch,
                                                    int rhs_mode, const double* __restrict__ c2f,
-                                                   double* __restrict__ divout, const EnsMember* __restrict__ M,
+                                                   double* __restrict__ divout, int div32, const EnsMember* __restrict__ M,
                                                    const int4* __restrict__ items, int nitems, int cur) {
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
@@ -325,20 +325,29 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const double Ci = T.ci[i], Dj = T.dj[j];
   const double dc = T.dc[ij], tdx = T.tdx[ij], tde = T.tde[ij], xy = T.xy[ij];
 
-  // Radial windows of p' and v_r over planes kb..kb+6.
-  double pw[7], rw[7];
+  // Radial windows of p' and v_r over planes kb..kb+6 (storage type).
+  S pw[7], rw[7];
   int kb = clampi(k0 - 3, 0, nr - 7);
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
-    pw[q] = static_cast<double>(f4[(size_t)(kb + q) * plane + ij]);
-    rw[q] = static_cast<double>(f3[(size_t)(kb + q) * plane + ij]);
+    pw[q] = f4[(size_t)(kb + q) * plane + ij];
+    rw[q] = f3[(size_t)(kb + q) * plane + ij];
   }
-  StagePlane<S> nxt;
+  // Planes in flight: one ahead for FP64 storage, two for FP32 (half the
+  // bytes per load, so twice the loads in flight for the same bandwidth).
+  constexpr bool kDeep = false;  // measured: the FP32 stage gained nothing from a second plane in flight
+  StagePlane<S> nxt, nxt2;
   load_plane(k0, nxt);
+  if (kDeep && k0 + 1 < k1) load_plane(k0 + 1, nxt2);
 
   for (int k = k0; k < k1; ++k) {
     const StagePlane<S> cur = nxt;
-    if (k + 1 < k1) load_plane(k + 1, nxt);
+    if constexpr (kDeep) {
+      nxt = nxt2;
+      if (k + 2 < k1) load_plane(k + 2, nxt2);
+    } else {
+      if (k + 1 < k1) load_plane(k + 1, nxt);
+    }
     const int want = clampi(k - 3, 0, nr - 7);
     if (want != kb) {  // slides by exactly one plane: plane k+3 enters
 #pragma unroll
@@ -401,7 +410,13 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     r4 = r4 - (q1 * row.g0) / row.rho0;
     // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
     const double c2rho0 = c2f ? c2f[(size_t)P * nf + (size_t)k * plane + ij] * row.rho0 : row.c2rho0;
-    if (divout) divout[(size_t)P * nf + (size_t)k * plane + ij] = div;
+    if (divout) {  // div(v) of the stage input, kept for the adjoint (FP64, or FP32 when div32)
+      const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
+      if (div32)
+        reinterpret_cast<float*>(divout)[dx] = static_cast<float>(div);
+      else
+        divout[dx] = div;
+    }
     double r5 = -(c2rho0 * div);
     r5 = r5 + row.rho0g0 * rc;
 
@@ -700,7 +715,7 @@ template <int TI, int TJ, typename S>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
-                                 double* divout) {
+                                 double* divout, int div32) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 2048, 32);
   const int kchunk = (T.nr + nkch - 1) / nkch;
@@ -708,13 +723,13 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   if (rhs_mode || stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
+                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
+                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, nullptr, nullptr, 0, 0);
+                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
   return hipGetLastError();
 }
 
@@ -726,7 +741,8 @@ static bool fits(int n, int ti, int tj) { return ti > 0 && tj > 0 && n % ti == 0
 template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                                 hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout) {
+                                 hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
+                                 int div32) {
   int ti = ls.stage_ti, tj = ls.stage_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 64x8 (n%64==0), 32x8, 16x16
     ti = pick_ti(T.n);
@@ -734,7 +750,8 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   }
 #define SOLAR_STAGE_CASE(A, B) \
   if (ti == A && tj == B)      \
-    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, \
+                                   div32);
   SOLAR_STAGE_CASE(64, 8)
   SOLAR_STAGE_CASE(64, 4)
   SOLAR_STAGE_CASE(32, 8)
@@ -744,20 +761,23 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
 #undef SOLAR_STAGE_CASE
   ti = pick_ti(T.n);
   if (ti == 64)
-    return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+    return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
+                                    div32);
   if (ti == 32)
-    return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
-  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+    return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
+                                    div32);
+  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
+                                   div32);
 }
 
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                        hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout) {
+                        hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout, int div32) {
   if (f32)
     return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
-                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32);
   return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
-                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout);
+                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32);
 }
 
 template <int TI, int TJ, typename S>
@@ -830,13 +850,13 @@ static hipError_t launch_stage_ens_t(const EnsMember* M, const int4* items, int 
   dim3 g(grid), block(TI, TJ);
   if (stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, M, items, nitems, cur);
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
   return hipGetLastError();
 }
 

@@ -44,6 +44,7 @@ EXPORTS = [
     ("solar_kernel_times", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     ("solar_set_c2_field", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_set_adjoint_options", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("solar_set_receivers", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("solar_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]),
     ("solar_adj_rhs", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
@@ -78,6 +79,7 @@ EXPORTS = [
     ("solar_ens_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("solar_ens_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _IP,
                                       ctypes.POINTER(ctypes.c_size_t)]),
+    ("solar_adjoint_times", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_last_error", ctypes.c_char_p, []),
     ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                          _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -364,6 +366,10 @@ class Solver:
             raise SolarError(f"c2 field of {c2.size} elements, expected shape {(6, self.nr, self.n, self.n)}")
         _check(self.lib.solar_set_c2_field(self.h, _p(c2)), "solar_set_c2_field")
 
+    def set_adjoint_options(self, div_bits=64):
+        """Storage of the stage divergences kept by misfit_gradient (64 exact, 32 half the memory)."""
+        _check(self.lib.solar_set_adjoint_options(self.h, int(div_bits)), "solar_set_adjoint_options")
+
     def set_receivers(self, pkji):
         """Receivers of p' at (panel, k, j, i), 0-based; array of shape (nrec, 4)."""
         a = np.ascontiguousarray(pkji, dtype=np.int32).reshape(-1, 4)
@@ -376,6 +382,12 @@ class Solver:
         out = np.zeros((int(nsteps), getattr(self, "nrec", 0)))
         _check(self.lib.solar_record(self.h, int(nsteps), float(dt), _p(out)), "solar_record")
         return out
+
+    def adjoint_times(self):
+        """Phases of the last misfit_gradient (s): workspace, forward, recompute, adjoint."""
+        out = np.zeros(4)
+        _check(self.lib.solar_adjoint_times(self.h, _p(out)), "solar_adjoint_times")
+        return dict(workspace_s=out[0], forward_s=out[1], recompute_s=out[2], adjoint_s=out[3])
 
     def adj_rhs(self, w):
         out = np.empty(self.shape)

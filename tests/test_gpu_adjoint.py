@@ -151,6 +151,13 @@ def test_misfit_gradient_matches_finite_differences():
         print(f"[adj] J0={J0:.6e}", flush=True)
         _, grad_one_segment, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=1000)
         assert np.allclose(grad, grad_one_segment, rtol=1e-12, atol=1e-12 * np.abs(grad).max())
+        # FP32 storage of the stage divergences: the gradient carries their rounding only
+        s.set_adjoint_options(32)
+        _, grad32, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=5)
+        s.set_adjoint_options(64)
+        rel32 = np.linalg.norm(grad32 - grad) / np.linalg.norm(grad)
+        print(f"[adj] FP32 divergences: rel L2 vs FP64 {rel32:.2e}", flush=True)
+        assert 0.0 < rel32 < 1e-6
 
         rng = np.random.default_rng(4)
         errs = []
