@@ -163,6 +163,14 @@ int solar_adj_rhs(solar_ctx* ctx, const double* w, double* out);
 int solar_adj_filter(solar_ctx* ctx, const double* o, double* out);
 int solar_adj_step(solar_ctx* ctx, double dt, const double* lam, double* out);
 
+/* Tangent-linear model of the receiver traces with respect to the c^2 field:
+ * dseis[n][r] = d p'_n(r) / d c2 . dc2 for nsteps RK4 steps from q0 (the same
+ * traces solar_record samples), run alongside the primal steps with the
+ * -dc2 rho0 div(v) source in every stage.  With the misfit gradient it gives
+ * the dot-product test sum 2 (seis - data) dseis = <dJ/dc2, dc2>.
+ * Overwrites the handle's state. */
+int solar_tangent_linear(solar_ctx* ctx, const double* q0, int nsteps, double dt, const double* dc2, double* dseis);
+
 /* Misfit J = sum over steps n and receivers r of (p'_n(r) - data[n][r])^2,
  * p'_n the state after step n+1 of nsteps RK4 steps from q0, and its
  * gradient grad(n,n,nr,6) = dJ/dc2 with respect to the c^2 field in use

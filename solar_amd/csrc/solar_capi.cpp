@@ -109,17 +109,24 @@ void release(solar_ctx* c) {
 // Enqueues one RK4 step (timestep.f90:61-82) on the handle's stream.  The
 // state lives in q[0] before and after: five stages ping-pong q[0] <-> q[1],
 // the filter brings it back.  With `timed`, events bracket every launch.
-int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = nullptr) {
+// divp: where stage s stores div(v) of its input.  b0, b1, bw: other state
+// and accumulator buffers (a tangent-linear state); tl_dc2 with tl_divp: the
+// tangent-linear source of each stage (primal divergences tl_divp[s]).
+int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = nullptr, void* b0 = nullptr,
+                 void* b1 = nullptr, void* bw = nullptr, const double* tl_dc2 = nullptr,
+                 double* const* tl_divp = nullptr) {
   const DevTables& T = c->dt;
-  void* cur = c->q[0];
-  void* nxt = c->q[1];
+  void* cur = b0 ? b0 : c->q[0];
+  void* nxt = b1 ? b1 : c->q[1];
+  void* qw = bw ? bw : c->qw;
   int e = 0;
   for (int s = 0; s < 5; ++s) {
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
     HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-    HIPCHK(solar::launch_stage(T, cur, nxt, c->qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
-                               c->shape, c->c2f, divp ? divp[s] : nullptr, c->div_bits == 32));
+    HIPCHK(solar::launch_stage(T, cur, nxt, qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
+                               c->shape, c->c2f, divp ? divp[s] : nullptr, c->div_bits == 32, tl_dc2,
+                               tl_dc2 ? tl_divp[s] : nullptr));
     void* t = cur;
     cur = nxt;
     nxt = t;
@@ -703,6 +710,42 @@ int solar_record(solar_ctx* c, int nsteps, double dt, double* seis) {
   }
   if (rc == SOLAR_OK) rc = download_f64(c, seis, d, count);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(v);
+  return rc;
+}
+
+int solar_tangent_linear(solar_ctx* c, const double* q0, int nsteps, double dt, const double* dc2, double* dseis) {
+  if (!c || !q0 || !dc2 || !dseis || nsteps < 1) return fail(SOLAR_ERR_ARG, "solar_tangent_linear: invalid argument");
+  if (c->nrec < 1) return fail(SOLAR_ERR_ARG, "solar_tangent_linear: no receivers set");
+  int rc = adj_alloc(c);
+  if (rc) return rc;
+  const size_t nq = c->nf * 30, nf6 = c->nf * 6, nrec = static_cast<size_t>(c->nrec);
+  const size_t nseis = static_cast<size_t>(nsteps) * nrec;
+  void* v = nullptr;
+  hipError_t e = hipMalloc(&v, (5 * nf6 + nf6 + nseis) * sizeof(double));
+  if (e != hipSuccess) return fail(SOLAR_ERR_HIP, std::string("solar_tangent_linear: hipMalloc: ") + hipGetErrorString(e));
+  double* divb = static_cast<double*>(v);
+  double* dc2d = divb + 5 * nf6;
+  double* ds = dc2d + nf6;
+  double* dp[5];
+  for (int s = 0; s < 5; ++s) dp[s] = divb + s * nf6;
+  const int bits = c->div_bits;
+  c->div_bits = 64;  // the tangent-linear source takes FP64 primal divergences
+  auto run = [&]() -> int {
+    int r;
+    if ((r = upload_state(c, c->q[0], q0, nq))) return r;
+    if ((r = upload_f64(c, dc2d, dc2, nf6))) return r;
+    HIPCHK(hipMemsetAsync(c->wb[0], 0, nq * sizeof(double), c->stream));
+    for (int n = 0; n < nsteps; ++n) {
+      if ((r = enqueue_step(c, dt, false, dp))) return r;  // primal, keeping its stage divergences
+      if ((r = enqueue_step(c, dt, false, nullptr, c->wb[0], c->wb[1], c->tmp, dc2d, dp))) return r;
+      HIPCHK(solar::launch_sample(c->wb[0], c->rec, c->nrec, ds + n * nrec, c->stream));
+    }
+    return download_f64(c, dseis, ds, nseis);
+  };
+  rc = run();
+  (void)hipStreamSynchronize(c->stream);
+  c->div_bits = bits;
   (void)hipFree(v);
   return rc;
 }

@@ -180,19 +180,23 @@ struct StagePlane {
 //   tile, panel, plane range) with a grid-sized stride; tables, buffers and
 //   dt come from the member (the state ping-pongs between M.q[cur] and
 //   M.q[1-cur]); the single-grid arguments are unused.
-template <int TI, int TJ, int KIND, typename S, bool ENS = false>
+//   MODE 2: tangent-linear stage (tl_dc2, tl_div used).
+template <int TI, int TJ, int KIND, typename S, int MODE = 0>
 __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Targ, const S* __restrict__ qin_a,
                                                    S* __restrict__ qout_a, S* __restrict__ qw_a,
                                                    const double* __restrict__ gxp_a, const double* __restrict__ gyp_a,
                                                    const double* __restrict__ gxv_a, const double* __restrict__ gyv_a,
                                                    double alpha, double beta, double dt_a, int kchunk, int nkch,
                                                    int rhs_mode, const double* __restrict__ c2f,
-                                                   double* __restrict__ divout, int div32, const EnsMember* __restrict__ M,
+                                                   double* __restrict__ divout, int div32,
+                                                   const double* __restrict__ tl_dc2, const double* __restrict__ tl_div,
+                                                   const EnsMember* __restrict__ M,
                                                    const int4* __restrict__ items, int nitems, int cur) {
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
   __shared__ double sve[2][TJ + 6][TI];
 
+  constexpr bool ENS = MODE == 1, TL = MODE == 2;
   const int n_items = ENS ? nitems : 1;
   for (int item = ENS ? (int)blockIdx.x : 0; item < n_items; item += ENS ? (int)gridDim.x : 1) {
   int bi, bj, P, k0, k1, mem = 0;
@@ -418,6 +422,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
         divout[dx] = div;
     }
     double r5 = -(c2rho0 * div);
+    // tangent-linear run: d r5 / d c^2 = -rho0 div of the primal stage input
+    if constexpr (TL) {
+      const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
+      r5 = r5 - (tl_dc2[dx] * row.rho0) * tl_div[dx];
+    }
     r5 = r5 + row.rho0g0 * rc;
 
     const bool do_bc = rhs_mode ? (rhs_mode & 1) : true;
@@ -715,21 +724,35 @@ template <int TI, int TJ, typename S>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
-                                 double* divout, int div32) {
+                                 double* divout, int div32, const double* tl_dc2, const double* tl_div) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 2048, 32);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
-  if (rhs_mode || stage == 0)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 0, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
-  else if (stage < 4)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 1, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
-  else
-    hipLaunchKernelGGL((k_stage<TI, TJ, 2, S>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, dt,
-                       kchunk, nkch, rhs_mode, c2f, divout, div32, nullptr, nullptr, 0, 0);
+  const int kind = (rhs_mode || stage == 0) ? 0 : (stage < 4 ? 1 : 2);
+#define SOLAR_STAGE_LAUNCH(K, M)                                                                                \
+  hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, \
+                     dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, nullptr, nullptr, 0, 0)
+  if (tl_dc2) {
+    if constexpr (sizeof(S) == 8) {
+      if (kind == 0)
+        SOLAR_STAGE_LAUNCH(0, 2);
+      else if (kind == 1)
+        SOLAR_STAGE_LAUNCH(1, 2);
+      else
+        SOLAR_STAGE_LAUNCH(2, 2);
+    } else {
+      return hipErrorInvalidValue;  // the tangent-linear stage is FP64 only
+    }
+  } else if (kind == 0) {
+    SOLAR_STAGE_LAUNCH(0, 0);
+  } else if (kind == 1) {
+    SOLAR_STAGE_LAUNCH(1, 0);
+  } else {
+    SOLAR_STAGE_LAUNCH(2, 0);
+  }
+#undef SOLAR_STAGE_LAUNCH
   return hipGetLastError();
 }
 
@@ -742,7 +765,7 @@ template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                                  hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
-                                 int div32) {
+                                 int div32, const double* tl_dc2, const double* tl_div) {
   int ti = ls.stage_ti, tj = ls.stage_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 64x8 (n%64==0), 32x8, 16x16
     ti = pick_ti(T.n);
@@ -751,7 +774,7 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
 #define SOLAR_STAGE_CASE(A, B) \
   if (ti == A && tj == B)      \
     return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, \
-                                   div32);
+                                   div32, tl_dc2, tl_div);
   SOLAR_STAGE_CASE(64, 8)
   SOLAR_STAGE_CASE(64, 4)
   SOLAR_STAGE_CASE(32, 8)
@@ -762,22 +785,23 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   ti = pick_ti(T.n);
   if (ti == 64)
     return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32);
+                                    div32, tl_dc2, tl_div);
   if (ti == 32)
     return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32);
+                                    div32, tl_dc2, tl_div);
   return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                   div32);
+                                   div32, tl_dc2, tl_div);
 }
 
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
-                        hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout, int div32) {
+                        hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout, int div32,
+                        const double* tl_dc2, const double* tl_div) {
   if (f32)
     return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
-                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32);
+                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div);
   return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
-                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32);
+                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div);
 }
 
 template <int TI, int TJ, typename S>
@@ -849,14 +873,17 @@ static hipError_t launch_stage_ens_t(const EnsMember* M, const int4* items, int 
   DevTables none{};
   dim3 g(grid), block(TI, TJ);
   if (stage == 0)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
+    hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nitems, cur);
   else if (stage < 4)
-    hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
+    hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nitems, cur);
   else
-    hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, true>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, M, items, nitems, cur);
+    hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nitems, cur);
   return hipGetLastError();
 }
 

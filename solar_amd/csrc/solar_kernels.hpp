@@ -52,11 +52,14 @@ hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, 
                          double* gys, double* gxv, double* gyv, hipStream_t st);
 // c2f (optional): a 3-D c^2 field (n,n,nr,6) used instead of the radial
 // profile; divout (optional): receives div(v) of the stage input (n,n,nr,6),
-// as double, or as float when div32 is set.
+// as double, or as float when div32 is set.  tl_dc2, tl_div (optional, both
+// or neither): tangent-linear stage, adding -dc2*rho0*div_primal to the
+// pressure equation (dc2 and div_primal fields (n,n,nr,6)).
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls, const double* c2f = nullptr,
-                        double* divout = nullptr, int div32 = 0);
+                        double* divout = nullptr, int div32 = 0, const double* tl_dc2 = nullptr,
+                        const double* tl_div = nullptr);
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
                          const double* fgy, hipStream_t st, const LaunchShape& ls);
 // Ensemble launches.  items: 4 ints per work item, (member, bi | bj << 16,

@@ -80,6 +80,7 @@ EXPORTS = [
     ("solar_ens_info", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t), _IP,
                                       ctypes.POINTER(ctypes.c_size_t)]),
     ("solar_adjoint_times", ctypes.c_int, [ctypes.c_void_p, _D]),
+    ("solar_tangent_linear", ctypes.c_int, [ctypes.c_void_p, _D, ctypes.c_int, ctypes.c_double, _D, _D]),
     ("solar_last_error", ctypes.c_char_p, []),
     ("solar_host_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.c_int, ctypes.c_double,
                                          _D, _D, _D, _D, _D, _D, _D, _D]),
@@ -381,6 +382,16 @@ class Solver:
         """nsteps RK4 steps, p' at the receivers after each: array (nsteps, nrec)."""
         out = np.zeros((int(nsteps), getattr(self, "nrec", 0)))
         _check(self.lib.solar_record(self.h, int(nsteps), float(dt), _p(out)), "solar_record")
+        return out
+
+    def tangent_linear(self, q0, nsteps, dt, dc2):
+        """d(traces)/d(c^2) . dc2, shape (nsteps, nrec)."""
+        dc2 = _f64(dc2)
+        if dc2.size != 6 * self.nr * self.n * self.n:
+            raise SolarError(f"dc2 of {dc2.size} values, expected shape {(6, self.nr, self.n, self.n)}")
+        out = np.zeros((int(nsteps), getattr(self, "nrec", 0)))
+        _check(self.lib.solar_tangent_linear(self.h, _p(self._arr(q0)), int(nsteps), float(dt), _p(dc2), _p(out)),
+               "solar_tangent_linear")
         return out
 
     def adjoint_times(self):

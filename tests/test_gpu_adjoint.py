@@ -201,3 +201,40 @@ def test_autograd_harness():
         J2, grad, _ = s.misfit_gradient(q0, cfg.steps, dt, data, checkpoint_every=4)
     assert abs(float(J) - J2) <= 1e-14 * J2
     assert np.allclose(c2.grad.numpy(), 3.0 * grad, rtol=1e-12, atol=1e-12 * np.abs(grad).max())
+
+
+def test_tangent_linear_dot_product():
+    """<2 (seis - d), TL dc2> = <dJ/dc2, dc2>: the tangent-linear traces and the
+    adjoint gradient agree to rounding (no finite-difference error); the
+    tangent-linear traces also match central differences of the traces."""
+    cfg, bg, dt, q0, rec, c2_0, c2_true = _gradient_problem()
+    steps = cfg.steps
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_receivers(rec)
+        s.set_c2_field(c2_true)
+        s.set_state(q0)
+        data = s.record(steps, dt)
+        s.set_c2_field(c2_0)
+        J, grad, seis = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=5)
+        rng = np.random.default_rng(21)
+        for t in range(3):
+            dc2 = rng.standard_normal(c2_0.shape) * c2_0
+            s.set_c2_field(c2_0)
+            dseis = s.tangent_linear(q0, steps, dt, dc2)
+            lhs = float(np.sum(2.0 * (seis - data) * dseis))
+            rhs = float(np.vdot(grad, dc2))
+            print(f"[adj] TL dot test {t}: <2r, TL dc2>={lhs:.15e} <grad, dc2>={rhs:.15e} "
+                  f"rel={abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
+            assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+            if t == 0:
+                eps = 1e-6
+                s.set_c2_field(c2_0 + eps * dc2)
+                s.set_state(q0)
+                sp = s.record(steps, dt)
+                s.set_c2_field(c2_0 - eps * dc2)
+                s.set_state(q0)
+                sm = s.record(steps, dt)
+                fd = (sp - sm) / (2 * eps)
+                rel = np.linalg.norm(fd - dseis) / np.linalg.norm(dseis)
+                print(f"[adj] TL traces vs central differences: rel L2 {rel:.2e}", flush=True)
+                assert rel < 1e-5
