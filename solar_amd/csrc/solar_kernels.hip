@@ -337,21 +337,14 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     pw[q] = f4[(size_t)(kb + q) * plane + ij];
     rw[q] = f3[(size_t)(kb + q) * plane + ij];
   }
-  // Planes in flight: one ahead for FP64 storage, two for FP32 (half the
-  // bytes per load, so twice the loads in flight for the same bandwidth).
-  constexpr bool kDeep = false;  // measured: the FP32 stage gained nothing from a second plane in flight
-  StagePlane<S> nxt, nxt2;
+  // The next plane is loaded while this one is computed (a second plane in
+  // flight was measured: no gain for FP64 or FP32 storage).
+  StagePlane<S> nxt;
   load_plane(k0, nxt);
-  if (kDeep && k0 + 1 < k1) load_plane(k0 + 1, nxt2);
 
   for (int k = k0; k < k1; ++k) {
     const StagePlane<S> cur = nxt;
-    if constexpr (kDeep) {
-      nxt = nxt2;
-      if (k + 2 < k1) load_plane(k + 2, nxt2);
-    } else {
-      if (k + 1 < k1) load_plane(k + 1, nxt);
-    }
+    if (k + 1 < k1) load_plane(k + 1, nxt);
     const int want = clampi(k - 3, 0, nr - 7);
     if (want != kb) {  // slides by exactly one plane: plane k+3 enters
 #pragma unroll
