@@ -132,6 +132,42 @@ def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
     return out
 
 
+def _c3_bench(steps, cpu):
+    """The 6x256x256x384 grid: GPU rate, stage-kernel roofline, and a one-step
+    CPU sample of the same grid (after a warm-up step)."""
+    from solar_amd import CONFIGS, Solver, make_inputs
+    cfg = CONFIGS["c3"]
+    bg, q0, dt, _ = make_inputs(cfg)
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        s.set_state(q0)
+        s.rk4(5, dt)
+        s.sync()
+        t = time.perf_counter()
+        s.rk4_async(steps, dt)
+        s.sync()
+        wall = time.perf_counter() - t
+        ok = s.check_finite()
+    os.environ["SOLAR_PROFILE"] = "1"
+    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    os.environ.pop("SOLAR_PROFILE")
+    p.set_state(q0)
+    p.rk4(2, dt)
+    p.rk4(5, dt)
+    kt = p.kernel_times()
+    p.close()
+    rate = cfg.points * steps / wall
+    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points
+    out = {"workload": DESCR["c3"], "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
+           "ms_per_step": wall / steps * 1e3, "step_frac": rate * STEP_BYTES / HBM_PEAK,
+           "stage_frac": stage_bytes / (kt["stage_ms"] * 1e-3) / HBM_PEAK, "state_finite": ok,
+           "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
+    if cpu:
+        c = _cpu_baseline(cfg, bg, q0, dt, 1)
+        out["cpu_baseline"] = c
+        out["gpu_over_cpu"] = rate / c["value"]
+    return out
+
+
 def _ensemble_bench(steps, reps):
     """Batched workload: 48 independent Gaussian-pulse simulations, uniform
     background, angular n and radial nr drawn from {32,48,64,96} x {64,128,192},
@@ -182,7 +218,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-sample-steps", type=int, default=1)
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 6x256x256x384 measurement (GPU rate, stage roofline, CPU sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--state-bits", type=int, default=64, choices=(64, 32),
@@ -271,6 +309,10 @@ def main():
             ["rho", "v_xi", "v_eta", "v_r", "p"], errs))}
         del fin
 
+    c3 = None
+    if not args.no_c3 and bits == 64 and args.workload != "c3":
+        c3 = _c3_bench(200, not args.no_cpu)
+
     ensemble = None
     if not args.no_ensemble and bits == 64:
         ensemble = _ensemble_bench(args.ensemble_steps, 3)
@@ -295,6 +337,8 @@ def main():
         out["adjoint"] = adjoint
     if ensemble:
         out["ensemble"] = ensemble
+    if c3:
+        out["c3"] = c3
     if cpu:
         out["gpu_over_cpu"] = rate / cpu["value"]
     print(json.dumps(out), flush=True)

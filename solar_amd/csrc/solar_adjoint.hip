@@ -95,7 +95,9 @@ __device__ __forceinline__ double c2rho0_at(const DevTables& T, const RadialRow&
 //   gc2  : c^2 gradient accumulator (n,n,nr,6)
 //   gxa_*, gya_* : dt-scaled ghost adjoints written by edge tiles
 // In RHS mode (mode & kAdjRhsOnly) Yout = L^T W and dt must be 1.
-template <int TI, int TJ>
+// D32: divs holds FP32 divergences (a template parameter, so the plane
+// prefetch holds a single load path).
+template <int TI, int TJ, bool D32>
 __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double* __restrict__ W,
                                                        const double* Yin, double* Yout,  // may alias
                                                        double* __restrict__ Wout, const double* __restrict__ divs,
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
                                                        double* __restrict__ gxa_p, double* __restrict__ gya_p,
                                                        double* __restrict__ gxa_v, double* __restrict__ gya_v,
                                                        double dt, double alpha, double beta, double wscale,
-                                                       int kchunk, int nkch, int mode, int div32) {
+                                                       int kchunk, int nkch, int mode) {
   // Transposed-stencil inputs with i halos (x) and j halos (y); two buffers
   // alternating by plane, so one barrier per plane suffices.
   __shared__ double sxv[2][TJ][TI + 6], sxp[2][TJ][TI + 6];
@@ -199,8 +201,10 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     }
     if (grad) {
       const size_t g = (size_t)P * nf + (size_t)k * plane + ij;
-      L.dv = div32 ? static_cast<double>(__builtin_nontemporal_load(reinterpret_cast<const float*>(divs) + g))
-                   : __builtin_nontemporal_load(divs + g);
+      if constexpr (D32)
+        L.dv = static_cast<double>(reinterpret_cast<const float*>(divs)[g]);
+      else
+        L.dv = __builtin_nontemporal_load(divs + g);
       L.g = gc2[g];
     }
     if (h_in) {
@@ -669,7 +673,7 @@ __global__ void k_inject(double* __restrict__ lam, const long long* __restrict__
 
 static int adj_ti(int n) { return (n % 32 == 0) ? 32 : 16; }
 
-template <int TI, int TJ>
+template <int TI, int TJ, bool D32>
 static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, hipStream_t st) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
   int nkch = (2048 + tiles - 1) / tiles;
@@ -677,15 +681,17 @@ static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, 
   if (nkch > maxk) nkch = maxk;
   if (nkch < 1) nkch = 1;
   const int kchunk = (T.nr + nkch - 1) / nkch;
-  hipLaunchKernelGGL((k_adj_stage<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout, a.Wout,
-                     a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, a.wscale,
-                     kchunk, nkch, a.mode, a.div32);
+  hipLaunchKernelGGL((k_adj_stage<TI, TJ, D32>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout,
+                     a.Wout, a.divs, a.gc2, a.c2f, a.gxa_p, a.gya_p, a.gxa_v, a.gya_v, a.dt, a.alpha, a.beta, a.wscale,
+                     kchunk, nkch, a.mode);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream_t st) {
-  if (adj_ti(T.n) == 32) return launch_adj_stage_t<32, 8>(T, a, st);
-  return launch_adj_stage_t<16, 16>(T, a, st);
+  const bool d32 = a.div32 && a.divs && (a.mode & kAdjGrad);
+  if (adj_ti(T.n) == 32)
+    return d32 ? launch_adj_stage_t<32, 8, true>(T, a, st) : launch_adj_stage_t<32, 8, false>(T, a, st);
+  return d32 ? launch_adj_stage_t<16, 16, true>(T, a, st) : launch_adj_stage_t<16, 16, false>(T, a, st);
 }
 
 hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
