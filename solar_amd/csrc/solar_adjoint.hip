@@ -191,6 +191,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
   const double* yi = Yin + (size_t)(P * 5) * nf;
   struct AdjPlane {
     double w[5], hw[4], rw[3], y[5], dv, g;
+    float dvf;  // D32: the divergence as loaded, widened where it is used (a conversion at
+                // the load would wait on every load in flight)
   };
   auto load_plane = [&](int k, AdjPlane& L) {
 #pragma unroll
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     if (grad) {
       const size_t g = (size_t)P * nf + (size_t)k * plane + ij;
       if constexpr (D32)
-        L.dv = static_cast<double>(reinterpret_cast<const float*>(divs)[g]);
+        L.dvf = __builtin_nontemporal_load(reinterpret_cast<const float*>(divs) + g);
       else
         L.dv = __builtin_nontemporal_load(divs + g);
       L.g = gc2[g];
@@ -346,7 +348,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       }
       if (grad) {
         // r5 = -c^2 rho0 div + ...  ->  d r5 / d c^2 = -rho0 div (zero on the BC planes)
-        gc2[(size_t)P * nf + x] = cur.g + dt * (-row.rho0 * cur.dv) * u.b5;
+        const double dv = D32 ? static_cast<double>(cur.dvf) : cur.dv;
+        gc2[(size_t)P * nf + x] = cur.g + dt * (-row.rho0 * dv) * u.b5;
       }
     }
 

@@ -156,6 +156,10 @@ struct StagePlane {
   S w[5];                      // qw at (i,j,k)
   double hpx, hvx;             // i-halo cell (p', v_xi)
   double hpy0, hve0, hpy1, hve1;  // low / high j-halo cells (p', v_eta)
+  // FP32 state: halo cells inside the panel, kept as loaded and widened when
+  // they go to LDS (a conversion at the load would wait on every load in
+  // flight); the double members then hold the ghost-strip cells only
+  S fpx, fvx, fpy0, fve0, fpy1, fve1;
   S pn, rn;                    // p', v_r of the plane entering the window
 };
 
@@ -166,9 +170,13 @@ struct StagePlane {
 // LDS tile, and plane k+1 is loaded into registers while plane k is computed.
 //   KIND 0: first stage (qw taken as 0, written), 1: middle stage (qw read and
 //   written), 2: last stage (qw read, not written back).
-// Occupancy floor of the stage kernel (waves per SIMD); 0 leaves it to the compiler.
+// Occupancy floor of the stage kernel (waves per SIMD); 0 leaves it to the
+// compiler.  2 keeps every variant within 256 registers: left alone, the
+// ensemble and tangent-linear middle stages take 254-255 VGPRs plus 2 AGPRs
+// and drop to one wave per SIMD (measured: ensemble step 44.3 -> 42.4 ms,
+// the single-grid stages unchanged).
 #ifndef SOLAR_STAGE_WAVES
-#define SOLAR_STAGE_WAVES 0
+#define SOLAR_STAGE_WAVES 2
 #endif
 #if SOLAR_STAGE_WAVES > 0
 #define SOLAR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(SOLAR_STAGE_WAVES)))
@@ -290,6 +298,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
   }
 
+  constexpr bool kWide = sizeof(S) == sizeof(double);
   auto load_plane = [&](int k, StagePlane<S>& L) {
     const size_t kp = (size_t)k * plane + ij;
     L.vx = f1[kp];
@@ -308,17 +317,47 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     for (int f = 0; f < 5; ++f)
       L.w[f] = (KIND == 0) ? S(0) : __builtin_nontemporal_load(w0 + f * nf + kp);
     const size_t kf = (size_t)k * plane, kg = (size_t)k * 3 * n;
-    if (hx) {
-      L.hpx = hx_g ? gxpW[kg + hx_off] : static_cast<double>(f4[kf + hx_off]);
-      L.hvx = hx_g ? gxvW[kg + hx_off] : static_cast<double>(f1[kf + hx_off]);
-    }
-    if (hy0) {
-      L.hpy0 = hy0_g ? gypS[kg + hy0_off] : static_cast<double>(f4[kf + hy0_off]);
-      L.hve0 = hy0_g ? gyvS[kg + hy0_off] : static_cast<double>(f2[kf + hy0_off]);
-    }
-    if (hy1) {
-      L.hpy1 = hy1_g ? gypS[kg + hy1_off] : static_cast<double>(f4[kf + hy1_off]);
-      L.hve1 = hy1_g ? gyvS[kg + hy1_off] : static_cast<double>(f2[kf + hy1_off]);
+    if constexpr (kWide) {
+      if (hx) {
+        L.hpx = hx_g ? gxpW[kg + hx_off] : f4[kf + hx_off];
+        L.hvx = hx_g ? gxvW[kg + hx_off] : f1[kf + hx_off];
+      }
+      if (hy0) {
+        L.hpy0 = hy0_g ? gypS[kg + hy0_off] : f4[kf + hy0_off];
+        L.hve0 = hy0_g ? gyvS[kg + hy0_off] : f2[kf + hy0_off];
+      }
+      if (hy1) {
+        L.hpy1 = hy1_g ? gypS[kg + hy1_off] : f4[kf + hy1_off];
+        L.hve1 = hy1_g ? gyvS[kg + hy1_off] : f2[kf + hy1_off];
+      }
+    } else {
+      if (hx) {
+        if (hx_g) {
+          L.hpx = gxpW[kg + hx_off];
+          L.hvx = gxvW[kg + hx_off];
+        } else {
+          L.fpx = f4[kf + hx_off];
+          L.fvx = f1[kf + hx_off];
+        }
+      }
+      if (hy0) {
+        if (hy0_g) {
+          L.hpy0 = gypS[kg + hy0_off];
+          L.hve0 = gyvS[kg + hy0_off];
+        } else {
+          L.fpy0 = f4[kf + hy0_off];
+          L.fve0 = f2[kf + hy0_off];
+        }
+      }
+      if (hy1) {
+        if (hy1_g) {
+          L.hpy1 = gypS[kg + hy1_off];
+          L.hve1 = gyvS[kg + hy1_off];
+        } else {
+          L.fpy1 = f4[kf + hy1_off];
+          L.fve1 = f2[kf + hy1_off];
+        }
+      }
     }
     // plane entering the radial window when the window slides before k
     const int kn = min(k + 3, nr - 1);
@@ -364,16 +403,16 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     svx[bf][ty][tx + 3] = cur.vx;
     sve[bf][ty + 3][tx] = cur.ve;
     if (hx) {
-      sp[bf][ty + 3][hx_col] = cur.hpx;
-      svx[bf][ty][hx_col] = cur.hvx;
+      sp[bf][ty + 3][hx_col] = kWide || hx_g ? cur.hpx : static_cast<double>(cur.fpx);
+      svx[bf][ty][hx_col] = kWide || hx_g ? cur.hvx : static_cast<double>(cur.fvx);
     }
     if (hy0) {
-      sp[bf][ty][tx + 3] = cur.hpy0;
-      sve[bf][ty][tx] = cur.hve0;
+      sp[bf][ty][tx + 3] = kWide || hy0_g ? cur.hpy0 : static_cast<double>(cur.fpy0);
+      sve[bf][ty][tx] = kWide || hy0_g ? cur.hve0 : static_cast<double>(cur.fve0);
     }
     if (hy1) {
-      sp[bf][ty + 6][tx + 3] = cur.hpy1;
-      sve[bf][ty + 6][tx] = cur.hve1;
+      sp[bf][ty + 6][tx + 3] = kWide || hy1_g ? cur.hpy1 : static_cast<double>(cur.fpy1);
+      sve[bf][ty + 6][tx] = kWide || hy1_g ? cur.hve1 : static_cast<double>(cur.fve1);
     }
     __syncthreads();
 
@@ -526,11 +565,32 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
       }
     }
   }
-  // Two planes in flight: nx1 holds plane kk+1, nx2 plane kk+2.
-  double nx1[NC], nx2[NC];
-  auto load_plane = [&](int kk, double (&dst)[NC]) {
+  // Two planes in flight: nx1 holds plane kk+1, nx2 plane kk+2.  With an
+  // FP32 state the field cells stay FP32 in flight (.f) and are widened when
+  // they go to LDS, next to the FP64 ghost-strip cells (.g): a conversion at
+  // the load would wait on every load in flight.
+  constexpr bool kWide = sizeof(S) == sizeof(double);
+  struct FPlane {
+    double g[NC];
+    S f[NC];
+  };
+  FPlane nx1, nx2;
+  auto load_plane = [&](int kk, FPlane& dst) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) dst[c] = cval[c] ? csrc[c].at(kk) : 0.0;
+    for (int c = 0; c < NC; ++c) {
+      if constexpr (kWide) {
+        dst.g[c] = cval[c] ? csrc[c].at(kk) : 0.0;
+      } else if (cval[c]) {
+        if (csrc[c].g)
+          dst.g[c] = csrc[c].g[(size_t)kk * csrc[c].stride];
+        else
+          dst.f[c] = csrc[c].f[(size_t)kk * csrc[c].stride];
+      }
+    }
+  };
+  auto widen = [&](const FPlane& L, int c) -> double {
+    if constexpr (kWide) return L.g[c];
+    return !cval[c] ? 0.0 : (csrc[c].g ? L.g[c] : static_cast<double>(L.f[c]));
   };
 
   double win[11];
@@ -541,17 +601,13 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   load_plane(kbeg, nx1);
   load_plane(kbeg + 1, nx2);
   for (int kk = kbeg; kk < kend; ++kk) {
-    double cur[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      cur[c] = nx1[c];
-      nx1[c] = nx2[c];
-    }
+    const FPlane cur = nx1;
+    nx1 = nx2;
     if (kk + 2 < kend) load_plane(kk + 2, nx2);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int e = tid + c * NT;
-      if (e < NCELL) (&sq[0][0])[e] = cur[c];
+      if (e < NCELL) (&sq[0][0])[e] = widen(cur, c);
     }
     __syncthreads();
     // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
