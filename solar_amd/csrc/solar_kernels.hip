@@ -22,6 +22,8 @@
 //               register window.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "solar_device.hpp"
 #include "solar_kernels.hpp"
 
@@ -40,19 +42,6 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
     default: return a * n + (n - 1 - l);
   }
 }
-
-// A plane-strided source of one value per radial plane: the state field
-// inside the panel (storage type S) or a gathered FP64 ghost strip beyond its
-// edge.
-template <typename S>
-struct Src {
-  const S* f = nullptr;
-  const double* g = nullptr;
-  size_t stride = 0;
-  __device__ __forceinline__ double at(int k) const {
-    return g ? g[(size_t)k * stride] : static_cast<double>(f[(size_t)k * stride]);
-  }
-};
 
 // Ghost strips of one receiver edge (receiver panel P, edge e, plane k) per
 // block, all three layers.  The sender's three source lines (rows/columns
@@ -541,56 +530,91 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   const int a = max(k0, kNstr), bnd = min(k1, nr - kNstr);
   if (a >= bnd) return;
 
-  // Tile cells this thread fills, with their source and plane stride.  Corner
-  // cells beyond the panel in both directions are never read and stay 0.
+  // Tile cells this thread fills: kind 1 a field cell (fp, plane stride
+  // n*n), 2 a ghost-strip cell (gp, plane stride 3n), 0 none (the corner
+  // cells beyond the panel in both directions are never read; they and the
+  // thread slots past the tile stay 0).  Every pointer is valid and every
+  // load is issued unconditionally (unused ones read a fixed in-bounds
+  // address with stride 0), so the loads carry no branches and the waits
+  // before their use count only what is still needed.
   const int tid = ty * TI + tx;
-  Src<S> csrc[NC];
-  bool cval[NC];
+  const S* fp[NC];
+  const double* gp[NC];
+  unsigned fst[NC], gst[NC];  // plane strides: n*n*nr < 2^32 for any grid that fits in HBM
+  int kind[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int e = tid + c * NT;
-    cval[c] = false;
+    kind[c] = 0;
+    fp[c] = g;
+    gp[c] = gxW;
+    fst[c] = 0;
+    gst[c] = 0;
     if (e < NCELL) {
       const int rr = e / (TI + 6), cc = e % (TI + 6);
       const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
       const bool jin = gj >= 0 && gj < n, iin = gi >= 0 && gi < n;
       if (jin && iin) {
-        csrc[c].f = g + (size_t)gj * n + gi; csrc[c].stride = plane; cval[c] = true;
+        fp[c] = g + (size_t)gj * n + gi;
+        fst[c] = (unsigned)plane;
+        kind[c] = 1;
       } else if (jin) {
-        csrc[c].g = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
-        csrc[c].stride = (size_t)n * 3; cval[c] = true;
+        gp[c] = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
+        gst[c] = 3u * n;
+        kind[c] = 2;
       } else if (iin) {
-        csrc[c].g = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
-        csrc[c].stride = (size_t)3 * n; cval[c] = true;
+        gp[c] = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
+        gst[c] = 3u * n;
+        kind[c] = 2;
       }
     }
   }
-  // Two planes in flight: nx1 holds plane kk+1, nx2 plane kk+2.  With an
-  // FP32 state the field cells stay FP32 in flight (.f) and are widened when
-  // they go to LDS, next to the FP64 ghost-strip cells (.g): a conversion at
-  // the load would wait on every load in flight.
+  // Two planes in flight in alternating buffers pa / pb: a plane is consumed
+  // whole by the LDS write that starts its step, and its buffer is refilled
+  // with the plane two ahead right there, so a load has the rest of its step
+  // and the whole next step to arrive (no register copies between
+  // iterations, which would wait on the loads).  With an FP32 state the field
+  // cells stay FP32 in flight (.f) and are widened when they go to LDS, next
+  // to the FP64 ghost-strip cells (.g): a conversion at the load would also
+  // wait on every load in flight.
   constexpr bool kWide = sizeof(S) == sizeof(double);
   struct FPlane {
     double g[NC];
     S f[NC];
   };
-  FPlane nx1, nx2;
+  if constexpr (kWide) {  // one FP64 source per cell
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (kind[c] == 1) {
+        gp[c] = reinterpret_cast<const double*>(fp[c]);
+        gst[c] = fst[c];
+      }
+  }
+  // the cell pointers select between two buffers, which hides their address
+  // space from the compiler: the loads name global memory explicitly
+  using GD = const __attribute__((address_space(1))) double*;
+  using GS = const __attribute__((address_space(1))) S*;
+  // FP32 state: the FP64 ghost-strip load only in waves that hold a ghost cell
+  // (a wave-uniform branch; tiles off the panel edges have none)
+  bool wave_g[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) wave_g[c] = kWide || __ballot(kind[c] == 2) != 0;
   auto load_plane = [&](int kk, FPlane& dst) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if constexpr (kWide) {
-        dst.g[c] = cval[c] ? csrc[c].at(kk) : 0.0;
-      } else if (cval[c]) {
-        if (csrc[c].g)
-          dst.g[c] = csrc[c].g[(size_t)kk * csrc[c].stride];
-        else
-          dst.f[c] = csrc[c].f[(size_t)kk * csrc[c].stride];
+        dst.g[c] = ((GD)gp[c])[(unsigned)kk * gst[c]];
+      } else {
+        dst.f[c] = ((GS)fp[c])[(unsigned)kk * fst[c]];
+        dst.g[c] = wave_g[c] ? ((GD)gp[c])[(unsigned)kk * gst[c]] : 0.0;
       }
     }
   };
   auto widen = [&](const FPlane& L, int c) -> double {
-    if constexpr (kWide) return L.g[c];
-    return !cval[c] ? 0.0 : (csrc[c].g ? L.g[c] : static_cast<double>(L.f[c]));
+    if constexpr (kWide) return kind[c] ? L.g[c] : 0.0;
+    const double fv = static_cast<double>(L.f[c]);
+    const double v = kind[c] == 1 ? fv : L.g[c];
+    return kind[c] ? v : 0.0;
   };
 
   double win[11];
@@ -598,17 +622,14 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   for (int q = 0; q < 11; ++q) win[q] = 0.0;
 
   const int kbeg = a - kNstr, kend = bnd + kNstr;  // kend - kbeg >= 11
-  load_plane(kbeg, nx1);
-  load_plane(kbeg + 1, nx2);
-  for (int kk = kbeg; kk < kend; ++kk) {
-    const FPlane cur = nx1;
-    nx1 = nx2;
-    if (kk + 2 < kend) load_plane(kk + 2, nx2);
+  // store: the step completes an output plane (every step from the 11th on)
+  auto step = [&](int kk, FPlane& cur, auto store) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int e = tid + c * NT;
       if (e < NCELL) (&sq[0][0])[e] = widen(cur, c);
     }
+    load_plane(min(kk + 2, kend - 1), cur);  // past the end: a plane that is loaded and not used
     __syncthreads();
     // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
     // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
@@ -632,14 +653,31 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
 #pragma unroll
     for (int q = 0; q < 10; ++q) win[q] = win[q + 1];
     win[10] = v;
-    const int kout = kk - kNstr;
-    if (kout >= a) {  // FILTER_R interior (filter.f90:682-693)
+    if constexpr (decltype(store)::value) {  // FILTER_R interior (filter.f90:682-693)
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < 11; ++q) s = s + T.filtr[q] * win[q];
-      o[(size_t)kout * plane + ij] = static_cast<S>(s);
+      o[(size_t)(kk - kNstr) * plane + ij] = static_cast<S>(s);
     }
+  };
+  // Branch-free memory traffic per iteration (a fixed warm-up, then pairs of
+  // storing steps, then at most one more), so the waits before each LDS
+  // write leave the other buffer's loads in flight.
+  using NoStore = std::integral_constant<bool, false>;
+  using Store = std::integral_constant<bool, true>;
+  FPlane pa, pb;
+  load_plane(kbeg, pa);
+  load_plane(kbeg + 1, pb);
+  int kk = kbeg;
+  for (; kk < kbeg + 2 * kNstr; kk += 2) {  // fill the radial window: planes a-5 .. a+4
+    step(kk, pa, NoStore());
+    step(kk + 1, pb, NoStore());
   }
+  for (; kk + 1 < kend; kk += 2) {
+    step(kk, pa, Store());
+    step(kk + 1, pb, Store());
+  }
+  if (kk < kend) step(kk, pa, Store());
 }
 
 template <int TI, int TJ, typename S>
