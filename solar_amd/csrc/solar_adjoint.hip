@@ -52,16 +52,15 @@ struct AdjU {
 // Transposed pointwise part of COMPUTE_RHS at one point (BC_RHS^T applied to
 // W first: BC zeroes the end planes of the RHS).  The metric quotients come
 // from tables (the transpose needs no bit-for-bit agreement with the forward
-// expression order, only the same operator up to rounding).
-__device__ __forceinline__ void adj_point(const DevTables& T, const RadialRow& row, int k, int i, int j,
+// expression order, only the same operator up to rounding): Ci = C(i),
+// Dj = D(j), mq = (dc/C, dc/D, XY/C, XY/D) at the point.  Row is a radial row
+// in any address space.
+template <typename Row>
+__device__ __forceinline__ void adj_point(const Row& row, bool bc, double Ci, double Dj, const double4& mq,
                                           double c2rho0, double w1, double w2, double w3, double w4, double w5,
                                           AdjU& u) {
-  const bool bc = (k == 0 || k == T.nr - 1);
   const double b1 = bc ? 0.0 : w1, b2 = bc ? 0.0 : w2, b3 = bc ? 0.0 : w3, b4 = bc ? 0.0 : w4,
                b5 = bc ? 0.0 : w5;
-  const size_t ij = (size_t)j * T.n + i;
-  const double Ci = T.ci[i], Dj = T.dj[j];
-  const double4 mq = *reinterpret_cast<const double4*>(T.adj_metric + ij * 4);  // dc/C, dc/D, XY/C, XY/D
   // r1 = -rho0 div - v_r drho0dr ; r5 = -c2rho0 div + rho0 g0 v_r
   const double divb = -row.rho0 * b1 - c2rho0 * b5;
   // r2..r4 = -grad/rho0
@@ -80,9 +79,13 @@ __device__ __forceinline__ void adj_point(const DevTables& T, const RadialRow& r
   u.b5 = b5;
 }
 
-__device__ __forceinline__ double c2rho0_at(const DevTables& T, const RadialRow& row, const double* c2f, int P,
-                                           int k, size_t ij) {
-  return c2f ? c2f[(size_t)P * T.n * T.n * T.nr + (size_t)k * T.n * T.n + ij] * row.rho0 : row.c2rho0;
+// The radial inputs (u_rv, u_rp) alone: they need no metric.
+template <typename Row>
+__device__ __forceinline__ void adj_radial(const Row& row, bool bc, double c2rho0, double w1, double w4, double w5,
+                                           double& uv, double& up) {
+  const double b1 = bc ? 0.0 : w1, b4 = bc ? 0.0 : w4, b5 = bc ? 0.0 : w5;
+  uv = -row.rho0 * b1 - c2rho0 * b5;
+  up = -b4 * row.inv_rho0;
 }
 
 }  // namespace
@@ -134,21 +137,21 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
   // W^(s), or wscale * Ybar when the stage starts from Ybar alone (W == nullptr)
   const double* w0 = (W ? W : Yin) + (size_t)(P * 5) * nf;
   const double ws = W ? 1.0 : wscale;
-  auto wat = [&](int f, int k, size_t cell) { return ws * w0[(size_t)f * nf + (size_t)k * plane + cell]; };
+  const CRadialRows rows = const_rows(T.rad);
+  auto c2rho0_of = [&](const auto& row, double c2) { return c2f ? c2 * row.rho0 : row.c2rho0; };
+  auto c2_at = [&](int k, size_t cell) { return c2f ? c2f[(size_t)P * nf + (size_t)k * plane + cell] : 0.0; };
 
-  // Radial inputs (u_rv, u_rp) of plane k at this column.
+  // Radial inputs (u_rv, u_rp) of plane k at this column, loaded on demand
+  // (the window fill and the closure planes).
   auto radial_u = [&](int k, double& uv, double& up) {
     if (k < 0 || k >= nr) {
       uv = 0.0;
       up = 0.0;
       return;
     }
-    const RadialRow& row = T.rad[k];
-    AdjU u;
-    adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), wat(0, k, ij), 0.0, 0.0, wat(3, k, ij),
-              wat(4, k, ij), u);
-    uv = u.urv;
-    up = u.urp;
+    const size_t c = (size_t)k * plane + ij;
+    adj_radial(rows[k], k == 0 || k == nr - 1, c2rho0_of(rows[k], c2_at(k, ij)), ws * w0[c], ws * w0[3 * nf + c],
+               ws * w0[4 * nf + c], uv, up);
   };
   // Radial window over planes k-3..k+3.
   double rv[7], rp[7];
@@ -181,56 +184,72 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
   const bool h_in = has_h && hi_ >= 0 && hi_ < n && hj_ >= 0 && hj_ < n;
   const size_t hcell = h_in ? (size_t)hj_ * n + hi_ : 0;
 
+  // Loop-invariant metric of the own point and, for the halo cell, the three
+  // factors its one transposed-stencil input needs: x halos take
+  // (dc/C, D, XY/C) for u_xv, u_xp; y halos (dc/D, XY/D, C) for u_yv, u_yp.
+  const double Ci = T.ci[i], Dj = T.dj[j];
+  const double4 mq = *reinterpret_cast<const double4*>(T.adj_metric + ij * 4);
+  const double tdx = T.tdx[ij], tde = T.tde[ij];
+  double hA = 0.0, hB = 0.0, hC = 0.0;
+  if (h_in) {
+    const double4 hm = *reinterpret_cast<const double4*>(T.adj_metric + hcell * 4);
+    hA = hx ? hm.x : hm.y;
+    hB = hx ? T.dj[hj_] : hm.w;
+    hC = hx ? hm.z : T.ci[hi_];
+  }
+
   const double sig_dt = (mode & kAdjRhsOnly) ? 1.0 : dt;
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
 
   // W values of one plane, loaded a plane ahead: own point, halo cell, and
-  // the plane entering the radial window (k+3 for plane k).
-  // Ybar, div(v) and the c^2 gradient at the own point come with them.
+  // the plane entering the radial window (k+3 for plane k); Ybar, div(v), the
+  // c^2 gradient and c^2 come with them.  Every load is issued whatever the
+  // mode (unused inputs read a fixed in-bounds address), so no branch sits
+  // between the loads and the waits before their use count exactly what is
+  // still in flight.
   const bool upd = !(mode & kAdjRhsOnly), grad = (mode & kAdjGrad) != 0;
-  const double* yi = Yin + (size_t)(P * 5) * nf;
+  const double* yi = upd ? Yin + (size_t)(P * 5) * nf : w0;
+  const double* dvb = grad ? divs : w0;
+  const double* gcb = grad ? gc2 : w0;
+  const double* c2b = c2f ? c2f + (size_t)P * nf : w0;
   struct AdjPlane {
-    double w[5], hw[4], rw[3], y[5], dv, g;
+    double w[5], hw[4], rw[3], y[5], dv, g, c2o, c2h, c2r;
     float dvf;  // D32: the divergence as loaded, widened where it is used (a conversion at
                 // the load would wait on every load in flight)
   };
   auto load_plane = [&](int k, AdjPlane& L) {
+    const size_t x = (size_t)k * plane + ij;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) L.w[f] = w0[(size_t)f * nf + (size_t)k * plane + ij];
-    if (upd) {
+    for (int f = 0; f < 5; ++f) L.w[f] = w0[(size_t)f * nf + x];
 #pragma unroll
-      for (int f = 0; f < 5; ++f) L.y[f] = __builtin_nontemporal_load(yi + (size_t)f * nf + (size_t)k * plane + ij);
-    }
-    if (grad) {
-      const size_t g = (size_t)P * nf + (size_t)k * plane + ij;
-      if constexpr (D32)
-        L.dvf = __builtin_nontemporal_load(reinterpret_cast<const float*>(divs) + g);
-      else
-        L.dv = __builtin_nontemporal_load(divs + g);
-      L.g = gc2[g];
-    }
-    if (h_in) {
-      const size_t hk = (size_t)k * plane + hcell;
-      L.hw[0] = w0[hk];
-      L.hw[1] = w0[nf + hk];
-      L.hw[2] = w0[2 * nf + hk];
-      L.hw[3] = w0[4 * nf + hk];
-    }
-    const int ke = k + 3;
-    if (k > k0 && ke < nr) {
-      const size_t ek = (size_t)ke * plane + ij;
-      L.rw[0] = w0[ek];
-      L.rw[1] = w0[3 * nf + ek];
-      L.rw[2] = w0[4 * nf + ek];
-    }
+    for (int f = 0; f < 5; ++f) L.y[f] = __builtin_nontemporal_load(yi + (size_t)f * nf + (upd ? x : 0));
+    const size_t g = grad ? (size_t)P * nf + x : 0;
+    if constexpr (D32)
+      L.dvf = __builtin_nontemporal_load(reinterpret_cast<const float*>(dvb) + g);
+    else
+      L.dv = __builtin_nontemporal_load(dvb + g);
+    L.g = gcb[g];
+    const size_t hk = (size_t)k * plane + hcell;
+    L.hw[0] = w0[hk];
+    L.hw[1] = w0[nf + hk];
+    L.hw[2] = w0[2 * nf + hk];
+    L.hw[3] = w0[4 * nf + hk];
+    const size_t ek = (size_t)min(k + 3, nr - 1) * plane + ij;
+    L.rw[0] = w0[ek];
+    L.rw[1] = w0[3 * nf + ek];
+    L.rw[2] = w0[4 * nf + ek];
+    L.c2o = c2b[c2f ? x : 0];
+    L.c2h = c2b[c2f ? hk : 0];
+    L.c2r = c2b[c2f ? ek : 0];
   };
   AdjPlane nxt;
   load_plane(k0, nxt);
 
   for (int k = k0; k < k1; ++k) {
     const AdjPlane cur = nxt;
-    if (k + 1 < k1) load_plane(k + 1, nxt);
-    const RadialRow& row = T.rad[k];
+    load_plane(min(k + 1, k1 - 1), nxt);  // the last plane's prefetch is loaded and not used
+    const auto& row = rows[k];
+    const bool bck = k == 0 || k == nr - 1;
     if (k > k0) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) {
@@ -239,12 +258,9 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       }
       const int ke = k + 3;
       if (ke < nr) {
-        const RadialRow& re = T.rad[ke];
-        AdjU ue;
-        adj_point(T, re, ke, i, j, c2rho0_at(T, re, c2f, P, ke, ij), ws * cur.rw[0], 0.0, 0.0, ws * cur.rw[1],
-                  ws * cur.rw[2], ue);
-        rv[6] = ue.urv;
-        rp[6] = ue.urp;
+        const auto& re = rows[ke];
+        adj_radial(re, ke == nr - 1, c2rho0_of(re, cur.c2r), ws * cur.rw[0], ws * cur.rw[1], ws * cur.rw[2], rv[6],
+                   rp[6]);
       } else {
         rv[6] = 0.0;
         rp[6] = 0.0;
@@ -255,19 +271,20 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     // own point
     const double w1 = ws * cur.w[0], w2 = ws * cur.w[1], w3 = ws * cur.w[2], w4 = ws * cur.w[3], w5 = ws * cur.w[4];
     AdjU u;
-    adj_point(T, row, k, i, j, c2rho0_at(T, row, c2f, P, k, ij), w1, w2, w3, w4, w5, u);
+    adj_point(row, bck, Ci, Dj, mq, c2rho0_of(row, cur.c2o), w1, w2, w3, w4, w5, u);
     sxv[bf][ty][tx + 3] = u.uxv;
     sxp[bf][ty][tx + 3] = u.uxp;
     syv[bf][ty + 3][tx] = u.uyv;
     syp[bf][ty + 3][tx] = u.uyp;
     if (has_h) {
       double hv = 0.0, hp = 0.0;
-      if (h_in) {
-        AdjU h;
-        adj_point(T, row, k, hi_, hj_, c2rho0_at(T, row, c2f, P, k, hcell), ws * cur.hw[0], ws * cur.hw[1],
-                  ws * cur.hw[2], 0.0, ws * cur.hw[3], h);
-        hv = hx ? h.uxv : h.uyv;
-        hp = hx ? h.uxp : h.uyp;
+      if (h_in) {  // the halo cell's one input: u_xv, u_xp (x halo) or u_yv, u_yp (y halo)
+        const double b1 = bck ? 0.0 : ws * cur.hw[0], b2 = bck ? 0.0 : ws * cur.hw[1],
+                     b3 = bck ? 0.0 : ws * cur.hw[2], b5 = bck ? 0.0 : ws * cur.hw[3];
+        const double divb = -row.rho0 * b1 - c2rho0_of(row, cur.c2h) * b5;
+        const double gxb = -b2 * row.inv_rho0, geb = -b3 * row.inv_rho0;
+        hv = row.inv_r * hA * divb;
+        hp = row.inv_r * (hB * gxb + hC * geb);
       }
       if (hx) {
         sxv[bf][hrow][hcol] = hv;
@@ -296,7 +313,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       if (kk >= 0 && kk < nr) {
         const int t = k - ddr_base(kk, nr);
         if (t >= 0 && t <= 6) {
-          const double wt = T.rad[kk].w[t];
+          const double wt = rows[kk].w[t];
           drv += wt * rv[q];
           drp += wt * rp[q];
         }
@@ -308,7 +325,7 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       for (int kk = 0; kk < k - 3; ++kk) {
         double uv, up;
         radial_u(kk, uv, up);
-        const double wt = T.rad[kk].w[k];
+        const double wt = rows[kk].w[k];
         drv += wt * uv;
         drp += wt * up;
       }
@@ -317,14 +334,13 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       for (int kk = k + 4; kk < nr; ++kk) {
         double uv, up;
         radial_u(kk, uv, up);
-        const double wt = T.rad[kk].w[k - (nr - 7)];
+        const double wt = rows[kk].w[k - (nr - 7)];
         drv += wt * uv;
         drp += wt * up;
       }
     }
 
     const double sg = row.sigma;
-    const double tdx = T.tdx[ij], tde = T.tde[ij];
     double lt[5];
     lt[0] = -(row.g0 * row.inv_rho0) * u.b4 - sg * w1;
     lt[1] = -row.inv_r * tdx * u.divb + dxv;

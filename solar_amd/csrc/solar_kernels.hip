@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict
 // ghost strips and are held as double.
 template <typename S>
 struct StagePlane {
-  S vx, ve, q1, pc, rc;        // v_xi, v_eta, rho', p', v_r at (i,j,k)
+  S vx, ve, q1;                // v_xi, v_eta, rho' at (i,j,k)
   S w[5];                      // qw at (i,j,k)
   double hpx, hvx;             // i-halo cell (p', v_xi)
   double hpy0, hve0, hpy1, hve1;  // low / high j-halo cells (p', v_eta)
@@ -150,6 +150,8 @@ struct StagePlane {
   // flight); the double members then hold the ghost-strip cells only
   S fpx, fvx, fpy0, fve0, fpy1, fve1;
   S pn, rn;                    // p', v_r of the plane entering the window
+  double c2;                   // optional 3-D c^2 at (i,j,k)
+  double tdc2, tdiv;           // tangent-linear inputs at (i,j,k)
 };
 
 // One RK stage (or one RHS evaluation when rhs_mode is set), one pass over
@@ -220,13 +222,13 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     if (k0 >= Targ.nr) return;
   }
   const DevTables& T = ENS ? M[mem].T : Targ;
-  const S* __restrict__ qin = ENS ? reinterpret_cast<const S*>(M[mem].q[cur]) : qin_a;
-  S* __restrict__ qout = ENS ? reinterpret_cast<S*>(M[mem].q[1 - cur]) : qout_a;
-  S* __restrict__ qw = ENS ? reinterpret_cast<S*>(M[mem].qw) : qw_a;
-  const double* __restrict__ gxp = ENS ? M[mem].gxp : gxp_a;
-  const double* __restrict__ gyp = ENS ? M[mem].gyp : gyp_a;
-  const double* __restrict__ gxv = ENS ? M[mem].gxv : gxv_a;
-  const double* __restrict__ gyv = ENS ? M[mem].gyv : gyv_a;
+  const auto qin = to_global(ENS ? reinterpret_cast<const S*>(M[mem].q[cur]) : qin_a);
+  const auto qout = to_global(ENS ? reinterpret_cast<S*>(M[mem].q[1 - cur]) : qout_a);
+  const auto qw = to_global(ENS ? reinterpret_cast<S*>(M[mem].qw) : qw_a);
+  const auto gxp = to_global(ENS ? M[mem].gxp : gxp_a);
+  const auto gyp = to_global(ENS ? M[mem].gyp : gyp_a);
+  const auto gxv = to_global(ENS ? M[mem].gxv : gxv_a);
+  const auto gyv = to_global(ENS ? M[mem].gyv : gyv_a);
   const double dt = ENS ? M[mem].dt : dt_a;
   const int n = T.n, nr = T.nr;
   const int tx = threadIdx.x, ty = threadIdx.y;
@@ -235,21 +237,21 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t ij = (size_t)j * n + i;
-  const S* f0 = qin + (size_t)(P * 5) * nf;  // rho'
-  const S* f1 = f0 + nf;                     // v_xi
-  const S* f2 = f1 + nf;                     // v_eta
-  const S* f3 = f2 + nf;                     // v_r
-  const S* f4 = f3 + nf;                     // p'
-  S* o0 = qout + (size_t)(P * 5) * nf;
-  S* w0 = qw + (size_t)(P * 5) * nf;
+  const auto f0 = qin + (size_t)(P * 5) * nf;  // rho'
+  const auto f1 = f0 + nf;                     // v_xi
+  const auto f2 = f1 + nf;                     // v_eta
+  const auto f3 = f2 + nf;                     // v_r
+  const auto f4 = f3 + nf;                     // p'
+  const auto o0 = qout + (size_t)(P * 5) * nf;
+  const auto w0 = qw + (size_t)(P * 5) * nf;
 
   // Halo cells this thread fills: source pointer and plane stride, from the
   // field inside the panel or from the gathered ghost strips beyond its edge.
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
-  const double* gxpW = gxp + (size_t)(P * 2) * gx_side;
-  const double* gxvW = gxv + (size_t)(P * 2) * gx_side;
-  const double* gypS = gyp + (size_t)(P * 2) * gy_side;
-  const double* gyvS = gyv + (size_t)(P * 2) * gy_side;
+  const auto gxpW = gxp + (size_t)(P * 2) * gx_side;
+  const auto gxvW = gxv + (size_t)(P * 2) * gx_side;
+  const auto gypS = gyp + (size_t)(P * 2) * gy_side;
+  const auto gyvS = gyv + (size_t)(P * 2) * gy_side;
   const bool hx = tx < 3 || tx >= TI - 3, hy0 = ty < 3, hy1 = ty >= TJ - 3;
   // Each halo slot: a 32-bit offset inside one plane of its source and
   // whether the source is a ghost strip (plane stride 3n) or the field
@@ -288,74 +290,71 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   }
 
   constexpr bool kWide = sizeof(S) == sizeof(double);
+  // The loads of a plane carry no branches (every halo slot reads a valid
+  // address: offset 0 of the plane when the thread has no such halo cell), so
+  // the waits before their use count exactly the stores issued after them.
+  // FP32 state: field halo cells load as FP32, ghost-strip cells as FP64 in
+  // waves that hold one (a wave-uniform branch).
+  const bool wg_x = kWide || __ballot(hx_g) != 0, wg_y0 = kWide || __ballot(hy0_g) != 0,
+             wg_y1 = kWide || __ballot(hy1_g) != 0;
+  const auto c2base = to_global(c2f ? c2f : T.ci);  // without c2f: a fixed in-bounds address
   auto load_plane = [&](int k, StagePlane<S>& L) {
     const size_t kp = (size_t)k * plane + ij;
     L.vx = f1[kp];
     L.ve = f2[kp];
     L.q1 = __builtin_nontemporal_load(f0 + kp);
-    // p' and v_r at plane k are the window centre on interior planes; the
-    // three end planes on each side read them here.
-    if (k < 3 || k >= nr - 3) {
-      L.pc = f4[kp];
-      L.rc = f3[kp];
-    } else {
-      L.pc = S(0);
-      L.rc = S(0);
-    }
 #pragma unroll
     for (int f = 0; f < 5; ++f)
       L.w[f] = (KIND == 0) ? S(0) : __builtin_nontemporal_load(w0 + f * nf + kp);
     const size_t kf = (size_t)k * plane, kg = (size_t)k * 3 * n;
     if constexpr (kWide) {
-      if (hx) {
-        L.hpx = hx_g ? gxpW[kg + hx_off] : f4[kf + hx_off];
-        L.hvx = hx_g ? gxvW[kg + hx_off] : f1[kf + hx_off];
-      }
-      if (hy0) {
-        L.hpy0 = hy0_g ? gypS[kg + hy0_off] : f4[kf + hy0_off];
-        L.hve0 = hy0_g ? gyvS[kg + hy0_off] : f2[kf + hy0_off];
-      }
-      if (hy1) {
-        L.hpy1 = hy1_g ? gypS[kg + hy1_off] : f4[kf + hy1_off];
-        L.hve1 = hy1_g ? gyvS[kg + hy1_off] : f2[kf + hy1_off];
-      }
+      L.hpx = hx_g ? gxpW[kg + hx_off] : f4[kf + hx_off];
+      L.hvx = hx_g ? gxvW[kg + hx_off] : f1[kf + hx_off];
+      L.hpy0 = hy0_g ? gypS[kg + hy0_off] : f4[kf + hy0_off];
+      L.hve0 = hy0_g ? gyvS[kg + hy0_off] : f2[kf + hy0_off];
+      L.hpy1 = hy1_g ? gypS[kg + hy1_off] : f4[kf + hy1_off];
+      L.hve1 = hy1_g ? gyvS[kg + hy1_off] : f2[kf + hy1_off];
     } else {
-      if (hx) {
-        if (hx_g) {
-          L.hpx = gxpW[kg + hx_off];
-          L.hvx = gxvW[kg + hx_off];
-        } else {
-          L.fpx = f4[kf + hx_off];
-          L.fvx = f1[kf + hx_off];
-        }
+      const int fx = hx_g ? 0 : hx_off, fy0 = hy0_g ? 0 : hy0_off, fy1 = hy1_g ? 0 : hy1_off;
+      L.fpx = f4[kf + fx];
+      L.fvx = f1[kf + fx];
+      L.fpy0 = f4[kf + fy0];
+      L.fve0 = f2[kf + fy0];
+      L.fpy1 = f4[kf + fy1];
+      L.fve1 = f2[kf + fy1];
+      if (wg_x) {
+        const int gx = hx_g ? hx_off : 0;
+        L.hpx = gxpW[kg + gx];
+        L.hvx = gxvW[kg + gx];
       }
-      if (hy0) {
-        if (hy0_g) {
-          L.hpy0 = gypS[kg + hy0_off];
-          L.hve0 = gyvS[kg + hy0_off];
-        } else {
-          L.fpy0 = f4[kf + hy0_off];
-          L.fve0 = f2[kf + hy0_off];
-        }
+      if (wg_y0) {
+        const int gy = hy0_g ? hy0_off : 0;
+        L.hpy0 = gypS[kg + gy];
+        L.hve0 = gyvS[kg + gy];
       }
-      if (hy1) {
-        if (hy1_g) {
-          L.hpy1 = gypS[kg + hy1_off];
-          L.hve1 = gyvS[kg + hy1_off];
-        } else {
-          L.fpy1 = f4[kf + hy1_off];
-          L.fve1 = f2[kf + hy1_off];
-        }
+      if (wg_y1) {
+        const int gy = hy1_g ? hy1_off : 0;
+        L.hpy1 = gypS[kg + gy];
+        L.hve1 = gyvS[kg + gy];
       }
     }
     // plane entering the radial window when the window slides before k
     const int kn = min(k + 3, nr - 1);
     L.pn = f4[(size_t)kn * plane + ij];
     L.rn = f3[(size_t)kn * plane + ij];
+    // per-point inputs of the RHS, prefetched with the plane: a load issued
+    // in the body would be waited on together with the whole prefetch
+    const size_t dx = (size_t)P * nf + kp;
+    L.c2 = c2base[c2f ? dx : 0];
+    if constexpr (TL) {
+      L.tdc2 = tl_dc2[dx];
+      L.tdiv = tl_div[dx];
+    }
   };
 
-  const double Ci = T.ci[i], Dj = T.dj[j];
-  const double dc = T.dc[ij], tdx = T.tdx[ij], tde = T.tde[ij], xy = T.xy[ij];
+  const double Ci = to_global(T.ci)[i], Dj = to_global(T.dj)[j];
+  const double dc = to_global(T.dc)[ij], tdx = to_global(T.tdx)[ij], tde = to_global(T.tde)[ij],
+               xy = to_global(T.xy)[ij];
 
   // Radial windows of p' and v_r over planes kb..kb+6 (storage type).
   S pw[7], rw[7];
@@ -372,7 +371,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
 
   for (int k = k0; k < k1; ++k) {
     const StagePlane<S> cur = nxt;
-    if (k + 1 < k1) load_plane(k + 1, nxt);
+    load_plane(min(k + 1, k1 - 1), nxt);  // the last plane's prefetch is loaded and not used
     const int want = clampi(k - 3, 0, nr - 7);
     if (want != kb) {  // slides by exactly one plane: plane k+3 enters
 #pragma unroll
@@ -384,9 +383,15 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
       rw[6] = cur.rn;
       kb = want;
     }
-    const bool interior = k >= 3 && k < nr - 3;
-    const double pc = interior ? pw[3] : cur.pc;
-    const double rc = interior ? rw[3] : cur.rc;
+    // p', v_r at plane k: window slot k - kb (3 on interior planes)
+    const int wk = k - kb;
+    S pcs = pw[0], rcs = rw[0];
+#pragma unroll
+    for (int q = 1; q < 7; ++q) {
+      pcs = wk == q ? pw[q] : pcs;
+      rcs = wk == q ? rw[q] : rcs;
+    }
+    const double pc = pcs, rc = rcs;
     const int bf = k & 1;
     sp[bf][ty + 3][tx + 3] = pc;
     svx[bf][ty][tx + 3] = cur.vx;
@@ -408,7 +413,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     // 7-point DRP stencils, ascending taps from 0 (derivative.f90:137-148,
     // 456-467, 656-668, 704-754).
     double dvx = 0.0, dve = 0.0, dpx = 0.0, dpe = 0.0, dvr = 0.0, dpr = 0.0;
-    const RadialRow& row = T.rad[k];
+    // the radial row is block-uniform and read-only: through the constant
+    // address space it comes in with scalar loads, which the vector-load
+    // waits do not count
+    const auto& row = const_rows(T.rad)[k];
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
       dvx = dvx + T.fcx[q] * svx[bf][ty][tx + q];
@@ -434,9 +442,9 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     double r4 = -(dpr / row.rho0);
     r4 = r4 - (q1 * row.g0) / row.rho0;
     // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
-    const double c2rho0 = c2f ? c2f[(size_t)P * nf + (size_t)k * plane + ij] * row.rho0 : row.c2rho0;
+    const double c2rho0 = c2f ? cur.c2 * row.rho0 : row.c2rho0;
+    const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
     if (divout) {  // div(v) of the stage input, kept for the adjoint (FP64, or FP32 when div32)
-      const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
       if (div32)
         reinterpret_cast<float*>(divout)[dx] = static_cast<float>(div);
       else
@@ -444,10 +452,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
     double r5 = -(c2rho0 * div);
     // tangent-linear run: d r5 / d c^2 = -rho0 div of the primal stage input
-    if constexpr (TL) {
-      const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
-      r5 = r5 - (tl_dc2[dx] * row.rho0) * tl_div[dx];
-    }
+    if constexpr (TL) r5 = r5 - (cur.tdc2 * row.rho0) * cur.tdiv;
     r5 = r5 + row.rho0g0 * rc;
 
     const bool do_bc = rhs_mode ? (rhs_mode & 1) : true;
