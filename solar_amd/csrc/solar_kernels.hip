@@ -365,12 +365,33 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     rw[q] = f3[(size_t)(kb + q) * plane + ij];
   }
   // The next plane is loaded while this one is computed (a second plane in
-  // flight was measured: no gain for FP64 or FP32 storage).
+  // flight was measured: no gain for FP64 or FP32 storage).  Each iteration
+  // ends by moving the prefetched registers into place, which waits for
+  // every vector-memory operation issued before it (the counter retires in
+  // issue order).  So a plane's results are held in registers and stored at
+  // the top of the next iteration, before its prefetch: by the end of that
+  // iteration the stores have had a whole plane's computation to retire.
   StagePlane<S> nxt;
   load_plane(k0, nxt);
+  S pq0, pq1, pq2, pq3, pq4, pw0, pw1, pw2, pw3, pw4;  // the previous plane's q, qw (or RHS)
+  size_t px = 0;
+  auto flush = [&]() {
+    const S pq[5] = {pq0, pq1, pq2, pq3, pq4}, pwv[5] = {pw0, pw1, pw2, pw3, pw4};
+    if (rhs_mode) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) o0[f * nf + px] = pq[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        __builtin_nontemporal_store(pq[f], o0 + f * nf + px);
+        if (KIND != 2) __builtin_nontemporal_store(pwv[f], w0 + f * nf + px);
+      }
+    }
+  };
 
   for (int k = k0; k < k1; ++k) {
     const StagePlane<S> cur = nxt;
+    if (k > k0) flush();
     load_plane(min(k + 1, k1 - 1), nxt);  // the last plane's prefetch is loaded and not used
     const int want = clampi(k - 3, 0, nr - 7);
     if (want != kb) {  // slides by exactly one plane: plane k+3 enters
@@ -467,25 +488,31 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
       r5 = r5 - sg * (pc - 0.0);
     }
 
-    const size_t x = (size_t)k * plane + ij;
+    px = (size_t)k * plane + ij;
     if (rhs_mode) {
-      o0[x] = static_cast<S>(r1);
-      o0[nf + x] = static_cast<S>(r2);
-      o0[2 * nf + x] = static_cast<S>(r3);
-      o0[3 * nf + x] = static_cast<S>(r4);
-      o0[4 * nf + x] = static_cast<S>(r5);
+      pq0 = static_cast<S>(r1);
+      pq1 = static_cast<S>(r2);
+      pq2 = static_cast<S>(r3);
+      pq3 = static_cast<S>(r4);
+      pq4 = static_cast<S>(r5);
     } else {
       // timestep.f90:68-69: qw = alpha*qw + dt*rhs ; q = q + beta*qw
-      const double rr[5] = {r1, r2, r3, r4, r5};
-      const double qq[5] = {q1, vx, ve, rc, pc};
-#pragma unroll
-      for (int f = 0; f < 5; ++f) {
-        const double wn = alpha * cur.w[f] + dt * rr[f];
-        __builtin_nontemporal_store(static_cast<S>(qq[f] + beta * wn), o0 + f * nf + x);
-        if (KIND != 2) __builtin_nontemporal_store(static_cast<S>(wn), w0 + f * nf + x);
-      }
+      const double w0n = alpha * cur.w[0] + dt * r1, w1n = alpha * cur.w[1] + dt * r2,
+                   w2n = alpha * cur.w[2] + dt * r3, w3n = alpha * cur.w[3] + dt * r4,
+                   w4n = alpha * cur.w[4] + dt * r5;
+      pq0 = static_cast<S>(q1 + beta * w0n);
+      pq1 = static_cast<S>(vx + beta * w1n);
+      pq2 = static_cast<S>(ve + beta * w2n);
+      pq3 = static_cast<S>(rc + beta * w3n);
+      pq4 = static_cast<S>(pc + beta * w4n);
+      pw0 = static_cast<S>(w0n);
+      pw1 = static_cast<S>(w1n);
+      pw2 = static_cast<S>(w2n);
+      pw3 = static_cast<S>(w3n);
+      pw4 = static_cast<S>(w4n);
     }
   }
+  flush();
   if constexpr (ENS) __syncthreads();  // the next item rewrites the LDS tiles
   }
 }
