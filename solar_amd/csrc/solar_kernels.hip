@@ -809,7 +809,7 @@ __global__ void k_place(const double* __restrict__ src, S* __restrict__ dst, siz
 
 // ---------------------------------------------------------------- launchers
 
-static int pick_ti(int n) { return (n % 64 == 0) ? 64 : ((n % 32 == 0) ? 32 : 16); }
+static int pick_ti(int n) { return (n % 32 == 0) ? 32 : 16; }
 
 int chunks_for(int tiles, int nr, int target_blocks, int min_chunk) {
   int nk = (target_blocks + tiles - 1) / tiles;
@@ -845,7 +845,8 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
                                  double* divout, int div32, const double* tl_dc2, const double* tl_div) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
-  const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 2048, 32);
+  // radial chunks of >= 32 planes, ~12k blocks: measured best at 6x128^2x256 and 6x256^2x384
+  const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 12288, 32);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
@@ -886,7 +887,7 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
                                  hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
                                  int div32, const double* tl_dc2, const double* tl_div) {
   int ti = ls.stage_ti, tj = ls.stage_tj;
-  if (!fits(T.n, ti, tj)) {  // measured best: 64x8 (n%64==0), 32x8, 16x16
+  if (!fits(T.n, ti, tj)) {  // measured best: 32x8 (64x8 within 2%), then 16x16
     ti = pick_ti(T.n);
     tj = ti == 64 ? 8 : 256 / ti;
   }
@@ -927,7 +928,9 @@ template <int TI, int TJ, typename S>
 static hipError_t launch_filter_t(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
                                   hipStream_t st, const LaunchShape& ls) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 30;
-  const int nkch = ls.filter_chunks > 0 ? ls.filter_chunks : chunks_for(tiles, T.nr, 2048, 48);
+  // radial chunks of >= 64 planes (each re-reads 10 halo planes), ~4k blocks: a single chunk per
+  // column at 6x256^2x384 leaves a long tail (3.45 ms against 2.96 ms with two)
+  const int nkch = ls.filter_chunks > 0 ? ls.filter_chunks : chunks_for(tiles, T.nr, 4096, 64);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
   hipLaunchKernelGGL((k_filter<TI, TJ, S>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
