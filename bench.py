@@ -248,7 +248,7 @@ def main():
                                                     args.checkpoint_every)}), flush=True)
         return
     if args.only == "ensemble":
-        print(json.dumps({"ensemble": _ensemble_bench(args.ensemble_steps, 3)}), flush=True)
+        print(json.dumps({"ensemble": _ensemble_bench(args.ensemble_steps, 10)}), flush=True)
         return
 
     from solar_amd import CONFIGS, Solver, make_inputs
@@ -315,7 +315,7 @@ def main():
 
     ensemble = None
     if not args.no_ensemble and bits == 64:
-        ensemble = _ensemble_bench(args.ensemble_steps, 3)
+        ensemble = _ensemble_bench(args.ensemble_steps, 10)
 
     adjoint = None
     if not args.no_adjoint and bits == 64:

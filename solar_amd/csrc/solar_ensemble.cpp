@@ -100,7 +100,7 @@ bool filter_shape_ok(int n, int ti, int tj) {
 
 Choice default_choice(int n, int nr) {
   Choice c;
-  c.sti = n % 64 == 0 ? 64 : (n % 32 == 0 ? 32 : 16);
+  c.sti = n % 32 == 0 ? 32 : 16;
   c.stj = c.sti == 16 ? 16 : 8;
   c.skc = std::min(nr, 32);
   c.fti = n % 32 == 0 ? 32 : 16;
@@ -374,17 +374,27 @@ int solar_ens_autotune(solar_ens* e, int reps) {
       }
     for (int m : kv.second) e->choice[m] = best;
   }
-  // then the grid policy: one block per item, or a persistent grid
-  float t_items = 0.f, t_pers = 0.f;
+  // The classes were timed one at a time, but a step launches them merged by
+  // shape: the tuned choices are kept only if the whole step beats the
+  // default shapes, which merge into the fewest launches.
+  auto step_ms = [&](float* ms) {
+    return time_launch(e, reps, [&]() { return enqueue_step(e) ? hipErrorUnknown : hipSuccess; }, ms);
+  };
   e->persist = 0;
+  float t_tuned = 0.f, t_default = 0.f;
+  if ((rc = build_lists(e)) || (rc = step_ms(&t_tuned))) return rc;
+  std::vector<Choice> tuned(e->choice.begin(), e->choice.end());
+  for (int m = 0; m < e->nm; ++m) e->choice[m] = default_choice(e->n[m], e->nr[m]);
+  if ((rc = build_lists(e)) || (rc = step_ms(&t_default))) return rc;
+  if (t_tuned < t_default) std::copy(tuned.begin(), tuned.end(), e->choice.begin());
+  // then the grid policy: one block per item, or a persistent grid
+  float t_items = std::min(t_tuned, t_default), t_pers = 0.f;
   if ((rc = build_lists(e))) return rc;
-  if ((rc = time_launch(e, reps, [&]() { return enqueue_step(e) ? hipErrorUnknown : hipSuccess; }, &t_items))) return rc;
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   e->persist = 4 * (cus > 0 ? cus : 256);
-  if ((rc = build_lists(e))) return rc;
-  if ((rc = time_launch(e, reps, [&]() { return enqueue_step(e) ? hipErrorUnknown : hipSuccess; }, &t_pers))) return rc;
+  if ((rc = build_lists(e)) || (rc = step_ms(&t_pers))) return rc;
   if (t_items <= t_pers) {
     e->persist = 0;
     if ((rc = build_lists(e))) return rc;
