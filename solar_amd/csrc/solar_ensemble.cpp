@@ -273,7 +273,7 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
     M.fgy = static_cast<double*>(p[8]);
     M.dt = dt[m];
     prefix[m] = e->gblocks;
-    e->gblocks += 24 * nr[m];
+    e->gblocks += 24 * solar::gather_blocks_per_edge(nr[m]);
     e->maxn = std::max(e->maxn, n[m]);
     e->choice[m] = default_choice(n[m], nr[m]);
   }

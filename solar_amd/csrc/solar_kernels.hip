@@ -43,95 +43,149 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
   }
 }
 
-// Ghost strips of one receiver edge (receiver panel P, edge e, plane k) per
-// block, all three layers.  The sender's three source lines (rows/columns
-// 2..4 or N-1..N-3 of grad.f90:28-31) are staged once in LDS -- for vectors
-// already converted to (theta, phi) with the sender's metric
+// Ghost strips of one receiver edge (receiver panel P, edge e) over the
+// planes [k0, k1), all three layers.  Per plane the sender's three source
+// lines (rows/columns 2..4 or N-1..N-3 of grad.f90:28-31) are staged in LDS
+// -- for vectors already converted to (theta, phi) with the sender's metric
 // (transform.f90:19-56); threads walk the layer index fastest so the three
 // adjacent columns of a W/E strip share cache lines.  Each thread then
 // interpolates ghost values along the edge with the 9-point Lagrange weights
 // (interpolation.f90:140-378; unit weights encode the end-point copies) and,
 // for vectors, applies the receiver-basis transform (transform.f90:161-274).
-// blockIdx.y selects a field offset (the filter's five scalars).
+// The conversion, weight and transform tables do not depend on the plane:
+// each thread loads its own once into registers.  The next plane's source
+// values load while this plane is interpolated, and the staged lines are
+// double-buffered in LDS, so one barrier per plane suffices.
 //   gx*: [P][side W/E][k][m][l],  gy*: [P][side S/N][k][l][m]
-//   edge = (plane k)*24 + (receiver panel P)*4 + (receiver edge e)
-template <bool VEC, typename S>
-__device__ __forceinline__ void gather_edge(const DevTables& T, int edge, int fld, const S* __restrict__ q,
-                                            int sfield, double* __restrict__ gxs, double* __restrict__ gys,
-                                            double* __restrict__ gxv, double* __restrict__ gyv,
-                                            long long field_stride_out, double* sline) {
+//   NI: strip points per thread (3n <= NI * blockDim.x)
+//   sline: 2 buffers x 3 layers x (3 quantities for vectors, else 1) x n
+template <bool VEC, typename S, int NI>
+__device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, int k0, int k1, int fld,
+                                            const S* __restrict__ q, int sfield, double* __restrict__ gxs,
+                                            double* __restrict__ gys, double* __restrict__ gxv,
+                                            double* __restrict__ gyv, long long field_stride_out, double* sline) {
+  constexpr int NQ = VEC ? 3 : 1;
   const int n = T.n, nr = T.nr;
-  const int k = edge / 24;
-  const int P = (edge % 24) / 4;
-  const int e = edge % 4;
-
   const Route rt = T.route[P * 4 + e];
   const int s = rt.sender, se = rt.sender_edge;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const S* sp = q + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
-  const S* vx = q + ((size_t)(s * 5 + 1) * nr + k) * plane;
-  const S* ve = vx + nf;
+  const auto sp = to_global(q + (size_t)(s * 5 + sfield + fld) * nf);
+  const auto vx = to_global(q + (size_t)(s * 5 + 1) * nf);
+  const auto ve = vx + nf;
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
-  const int row = (se == kW || se == kE);  // column strips: layer index is the fast memory index
-  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    const int l = row ? t % 3 + 1 : t / n + 1;
-    const int a = row ? t / 3 : t % n;
-    const int idx = strip_index(se, l, a, n);
-    double* ln = sline + (size_t)(l - 1) * 3 * n;
-    ln[a] = static_cast<double>(sp[idx]);
-    if (VEC) {
-      const double* c = cv + (size_t)idx * 6;
-      const double x = static_cast<double>(vx[idx]), y = static_cast<double>(ve[idx]);
-      ln[n + a] = c[0] * (c[1] * x + c[2] * y);
-      ln[2 * n + a] = c[3] * (c[4] * x + c[5] * y);
-    }
-  }
-  __syncthreads();
+  const int colstrip = (se == kW || se == kE);  // column strips: layer index is the fast memory index
+  const int lstride = NQ * n, bstride = 3 * lstride;
   const size_t fo = (size_t)fld * field_stride_out;
-  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    const int l = (e < 2) ? t % 3 + 1 : t / n + 1;  // output layout: l fastest for x ghosts
-    const int m = (e < 2) ? t / 3 : t % n;
-    const double* ln = sline + (size_t)(l - 1) * 3 * n;
-    const int mp = rt.reverse ? n - 1 - m : m;
-    const double* w = T.gw + ((size_t)(l - 1) * n + mp) * 9;
-    double ps = 0.0, ths = 0.0, phs = 0.0;
+
+  // This thread's staged points (l, a) and its outputs (l, m), with their
+  // plane-independent tables.
+  int sidx[NI], sslot[NI];
+  bool sval[NI], oval[NI];
+  double c[NI][6];
+  int obase_l[NI], omp[NI];
+  size_t obase[NI];
+  double w[NI][9], ab[NI][2];
 #pragma unroll
-    for (int tp = 0; tp < 9; ++tp) {
-      const int a = clampi(mp - 4 + tp, 0, n - 1);
-      const double wt = w[tp];
-      ps = ps + ln[a] * wt;
+  for (int it = 0; it < NI; ++it) {
+    const int t = threadIdx.x + it * blockDim.x;
+    sval[it] = t < 3 * n;
+    oval[it] = sval[it];
+    const int tt = sval[it] ? t : 0;
+    const int l = colstrip ? tt % 3 + 1 : tt / n + 1;
+    const int a = colstrip ? tt / 3 : tt % n;
+    sidx[it] = strip_index(se, l, a, n);
+    sslot[it] = (l - 1) * lstride + a;
+    if (VEC) {
+#pragma unroll
+      for (int x = 0; x < 6; ++x) c[it][x] = cv[(size_t)sidx[it] * 6 + x];
+    }
+    const int lo = (e < 2) ? tt % 3 + 1 : tt / n + 1;  // output layout: l fastest for x ghosts
+    const int m = (e < 2) ? tt / 3 : tt % n;
+    const int mp = rt.reverse ? n - 1 - m : m;
+    obase_l[it] = (lo - 1) * lstride;
+    omp[it] = mp;
+#pragma unroll
+    for (int tp = 0; tp < 9; ++tp) w[it][tp] = T.gw[((size_t)(lo - 1) * n + mp) * 9 + tp];
+    if (VEC) {
+      const double* abp = T.trans + ((((size_t)s * 4 + se) * 3 + (lo - 1)) * n + mp) * 2;
+      ab[it][0] = abp[0];
+      ab[it][1] = abp[1];
+    }
+    obase[it] = (e < 2) ? fo + ((size_t)(P * 2 + e) * nr * n + m) * 3 + (lo - 1)
+                        : fo + ((size_t)(P * 2 + (e - 2)) * nr * 3 + (lo - 1)) * n + m;
+  }
+  const auto gs = to_global(e < 2 ? gxs : gys);
+  const auto gv = to_global(e < 2 ? gxv : gyv);
+
+  // source values of one plane, in the storage type until staged
+  S pv[NI], xv[NI], yv[NI];
+  auto load = [&](int k) {
+    const size_t kp = (size_t)k * plane;
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      pv[it] = sp[kp + sidx[it]];
       if (VEC) {
-        ths = ths + ln[n + a] * wt;
-        phs = phs + ln[2 * n + a] * wt;
+        xv[it] = vx[kp + sidx[it]];
+        yv[it] = ve[kp + sidx[it]];
       }
     }
-    size_t o;
-    if (e < 2) {
-      o = fo + (((size_t)(P * 2 + e) * nr + k) * n + m) * 3 + (l - 1);
-      gxs[o] = ps;
-    } else {
-      o = fo + (((size_t)(P * 2 + (e - 2)) * nr + k) * 3 + (l - 1)) * n + m;
-      gys[o] = ps;
+  };
+  load(k0);
+  for (int k = k0; k < k1; ++k) {
+    double* buf = sline + ((k - k0) & 1) * bstride;
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!sval[it]) continue;
+      double* ln = buf + sslot[it];
+      ln[0] = static_cast<double>(pv[it]);
+      if (VEC) {
+        const double x = static_cast<double>(xv[it]), y = static_cast<double>(yv[it]);
+        ln[n] = c[it][0] * (c[it][1] * x + c[it][2] * y);
+        ln[2 * n] = c[it][3] * (c[it][4] * x + c[it][5] * y);
+      }
     }
-    if (VEC) {
-      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
-      const double g = ab[0] * ths + ab[1] * phs;
-      if (e < 2)
-        gxv[o] = g;
-      else
-        gyv[o] = g;
+    load(min(k + 1, k1 - 1));  // past the last plane: loaded and not used
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!oval[it]) continue;
+      const double* ln = buf + obase_l[it];
+      double ps = 0.0, ths = 0.0, phs = 0.0;
+#pragma unroll
+      for (int tp = 0; tp < 9; ++tp) {
+        const int a = clampi(omp[it] - 4 + tp, 0, n - 1);
+        const double wt = w[it][tp];
+        ps = ps + ln[a] * wt;
+        if (VEC) {
+          ths = ths + ln[n + a] * wt;
+          phs = phs + ln[2 * n + a] * wt;
+        }
+      }
+      const size_t o = obase[it] + (size_t)k * 3 * n;
+      gs[o] = ps;
+      if (VEC) gv[o] = ab[it][0] * ths + ab[it][1] * phs;
     }
   }
 }
 
-template <bool VEC, typename S>
-__global__ __launch_bounds__(256) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
-                                                double* __restrict__ gxs, double* __restrict__ gys,
-                                                double* __restrict__ gxv, double* __restrict__ gyv,
-                                                long long field_stride_out) {
-  extern __shared__ double sline[];  // [3 layers][3 quantities][n]: scalar, theta, phi
-  gather_edge<VEC, S>(T, blockIdx.x, blockIdx.y, q, sfield, gxs, gys, gxv, gyv, field_stride_out, sline);
+// blockIdx.x = (chunk of planes)*24 + (receiver panel P)*4 + (receiver edge e);
+// blockIdx.y selects a field offset (the filter's five scalars).
+// Threads per gather block: one strip point each up to 1024 (n <= 341), else
+// two each (n <= 512).
+constexpr int gather_threads_max(int ni) { return ni == 1 ? 1024 : 768; }
+inline int gather_threads(int n, int ni) { return (((3 * n + ni - 1) / ni) + 63) / 64 * 64; }
+
+template <bool VEC, typename S, int NI>
+__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
+                                                 double* __restrict__ gxs, double* __restrict__ gys,
+                                                 double* __restrict__ gxv, double* __restrict__ gyv,
+                                                 long long field_stride_out) {
+  extern __shared__ double sline[];
+  const int kc = gather_kc(T.nr);
+  const int k0 = (blockIdx.x / 24) * kc, edge = blockIdx.x % 24;
+  gather_edge<VEC, S, NI>(T, edge / 4, edge % 4, k0, min(k0 + kc, T.nr), blockIdx.y, q, sfield, gxs, gys, gxv, gyv,
+                          field_stride_out, sline);
 }
 
 // Values one thread loads for one radial plane of the stage kernel: its own
@@ -748,10 +802,11 @@ __global__ __launch_bounds__(TI* TJ) void k_filter_ens(const EnsMember* __restri
 }
 
 // Ensemble gather: the blocks of all members laid end to end (prefix[m] =
-// first block of member m, 24*nr blocks each); blockIdx.y selects the field.
-template <bool VEC>
-__global__ __launch_bounds__(256) void k_gather_ens(const EnsMember* __restrict__ M, const int* __restrict__ prefix,
-                                                    int nmem, int cur) {
+// first block of member m, 24 * ceil(nr / gather_kc(nr)) blocks each);
+// blockIdx.y selects the field.
+template <bool VEC, int NI>
+__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather_ens(const EnsMember* __restrict__ M, const int* __restrict__ prefix,
+                                                     int nmem, int cur) {
   extern __shared__ double sline[];
   const int b = blockIdx.x;
   int lo = 0, hi = nmem - 1;  // last member whose first block is <= b
@@ -763,12 +818,15 @@ __global__ __launch_bounds__(256) void k_gather_ens(const EnsMember* __restrict_
       hi = mid - 1;
   }
   const EnsMember& E = M[lo];
-  const long long fs = 12LL * E.T.nr * E.T.n * 3;
+  const int li = b - prefix[lo], nr = E.T.nr, kc = gather_kc(nr);
+  const int k0 = (li / 24) * kc, edge = li % 24, k1 = min(k0 + kc, nr);
+  const long long fs = 12LL * nr * E.T.n * 3;
   if (VEC)
-    gather_edge<true, double>(E.T, b - prefix[lo], blockIdx.y, E.q[cur], 4, E.gxp, E.gyp, E.gxv, E.gyv, fs, sline);
+    gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], 4, E.gxp, E.gyp, E.gxv,
+                                  E.gyv, fs, sline);
   else
-    gather_edge<false, double>(E.T, b - prefix[lo], blockIdx.y, E.q[cur], 0, E.fgx, E.fgy, nullptr, nullptr, fs,
-                               sline);
+    gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], 0, E.fgx, E.fgy, nullptr,
+                                   nullptr, fs, sline);
 }
 
 // Non-finite scan and v_r sum of squares; acc[0] counts non-finite values,
@@ -819,18 +877,37 @@ int chunks_for(int tiles, int nr, int target_blocks, int min_chunk) {
   return nk;
 }
 
+// Dynamic LDS of one gather launch; above 64 KiB the kernel is allowed it once.
+template <typename K>
+static hipError_t gather_lds(K kernel, size_t lds) {
+  if (lds <= 64 * 1024) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(lds));
+}
+
+template <bool VEC, typename S, int NI>
+static hipError_t launch_gather_v(const DevTables& T, const S* q, int sfield, int nfields, double* gxs, double* gys,
+                                  double* gxv, double* gyv, hipStream_t st) {
+  const long long field_stride = 12LL * T.nr * T.n * 3;
+  const int kc = gather_kc(T.nr);
+  dim3 grid((unsigned)(24 * ((T.nr + kc - 1) / kc)), (unsigned)nfields);
+  const int nt = gather_threads(T.n, NI);
+  const size_t lds = sizeof(double) * 2 * 3 * (VEC ? 3 : 1) * (size_t)T.n;  // n <= 512: at most 72 KiB
+  hipError_t e = gather_lds(&k_gather<VEC, S, NI>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_gather<VEC, S, NI>), grid, dim3(nt), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+  return hipGetLastError();
+}
+
 template <typename S>
 static hipError_t launch_gather_t(const DevTables& T, const S* q, bool vec, int sfield, int nfields, double* gxs,
                                   double* gys, double* gxv, double* gyv, hipStream_t st) {
-  const long long field_stride = 12LL * T.nr * T.n * 3;
-  dim3 grid((unsigned)(24 * T.nr), (unsigned)nfields);
-  const size_t lds = (size_t)9 * T.n * sizeof(double);
+  const bool two = 3 * T.n > 1024;  // n > 341: two strip points per thread
   if (vec)
-    hipLaunchKernelGGL((k_gather<true, S>), grid, dim3(256), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
-  else
-    hipLaunchKernelGGL((k_gather<false, S>), grid, dim3(256), lds, st, T, q, sfield, gxs, gys, gxv, gyv,
-                       field_stride);
-  return hipGetLastError();
+    return two ? launch_gather_v<true, S, 2>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st)
+               : launch_gather_v<true, S, 1>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st);
+  return two ? launch_gather_v<false, S, 2>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st)
+             : launch_gather_v<false, S, 1>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st);
 }
 
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
@@ -1035,14 +1112,25 @@ hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, i
   return hipGetLastError();
 }
 
+template <bool VEC, int NI>
+static hipError_t launch_gather_ens_v(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, int cur,
+                                      hipStream_t st) {
+  const int nt = gather_threads(maxn, NI);
+  const size_t lds = sizeof(double) * 2 * 3 * (VEC ? 3 : 1) * (size_t)maxn;
+  hipError_t e = gather_lds(&k_gather_ens<VEC, NI>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_gather_ens<VEC, NI>), dim3(nblocks, VEC ? 1 : 5), dim3(nt), lds, st, M, prefix, nmem, cur);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
                              hipStream_t st) {
-  const size_t lds = sizeof(double) * 9 * (size_t)maxn;
+  const bool two = 3 * maxn > 1024;  // n > 341: two strip points per thread
   if (vec)
-    hipLaunchKernelGGL(k_gather_ens<true>, dim3(nblocks, 1), dim3(256), lds, st, M, prefix, nmem, cur);
-  else
-    hipLaunchKernelGGL(k_gather_ens<false>, dim3(nblocks, 5), dim3(256), lds, st, M, prefix, nmem, cur);
-  return hipGetLastError();
+    return two ? launch_gather_ens_v<true, 2>(M, prefix, nmem, nblocks, maxn, cur, st)
+               : launch_gather_ens_v<true, 1>(M, prefix, nmem, nblocks, maxn, cur, st);
+  return two ? launch_gather_ens_v<false, 2>(M, prefix, nmem, nblocks, maxn, cur, st)
+             : launch_gather_ens_v<false, 1>(M, prefix, nmem, nblocks, maxn, cur, st);
 }
 
 }  // namespace solar

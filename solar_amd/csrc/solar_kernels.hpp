@@ -70,11 +70,15 @@ hipError_t launch_stage_ens(const EnsMember* M, const int* items, int nitems, in
                             int cur, hipStream_t st);
 hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int cur,
                              hipStream_t st);
-// prefix[m]: first block of member m (24*nr blocks each), nblocks in total;
-// maxn: the largest n (LDS size).  vec: the stage ghosts (p', velocity),
-// else the filter's five scalars.
+// prefix[m]: first block of member m (24 * gather_blocks_per_edge(nr) blocks
+// each), nblocks in total; maxn: the largest n (block size and LDS).  vec:
+// the stage ghosts (p', velocity), else the filter's five scalars.
 hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
                              hipStream_t st);
+// Planes per gather block (about 32 chunks per receiver edge), and gather
+// blocks per receiver edge.
+constexpr int gather_kc(int nr) { return (nr + 31) / 32; }
+constexpr int gather_blocks_per_edge(int nr) { return (nr + gather_kc(nr) - 1) / gather_kc(nr); }
 // mode 0: count of non-finite values -> acc[0]; mode 1: sum of squares of
 // `field` over all panels -> acc[1].
 hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st, int field = 3);

@@ -192,3 +192,24 @@ def test_rk4_tile_shapes(n, nr, kind, steps):
         gx, gy = s.vector_ghosts(q[:, 1], q[:, 2])
         ox, oy = o.vector_ghosts(q[:, 1], q[:, 2])
         assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+
+
+@pytest.mark.parametrize("n,nr", [(352, 13), (64, 45)])
+def test_ghosts_plane_chunks(n, nr):
+    """Ghost gathers with two strip points per thread (3n > 1024) and with a
+    ragged last chunk of planes (nr not a multiple of the planes per block)."""
+    from solar_amd import background
+    bg = background(1, nr)
+    print(f"[case] ghosts n={n} nr={nr}", flush=True)
+    rng = np.random.default_rng(n + nr)
+    f = rng.standard_normal((6, nr, n, n))
+    vxi = rng.standard_normal((6, nr, n, n))
+    veta = rng.standard_normal((6, nr, n, n))
+    o = oracle.Oracle(n, nr, *bg, 2, 0.1)
+    with Solver(n, nr, *bg, 2, 0.1) as s:
+        gx, gy = s.scalar_ghosts(f)
+        ox, oy = o.scalar_ghosts(f)
+        assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+        gx, gy = s.vector_ghosts(vxi, veta)
+        ox, oy = o.vector_ghosts(vxi, veta)
+        assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
