@@ -419,26 +419,36 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
 }
 
 // Transpose of k_gather, as a gather over the sender's strips: one block per
-// (sender panel s, sender edge se, plane k).  The receiver's ghost values of
-// that strip (receiver from the inverse route) are first staged in LDS in
-// the sender's along-edge order -- for vectors already through the
-// receiver-basis transform, as theta and phi parts -- then each strip point
-// (layer l, along index a) sums the receiver stencils that read it (the
-// transposed rows gt_*), applies the sender's (xi,eta)->(theta,phi)
-// conversion transposed for vectors, and adds the result to Y (and beta
-// times it to Wn).  pass 0 covers the S/N strips, pass 1 the W/E strips:
-// strips of one pass never share a point, so the adds need no atomics;
-// corner points get their two strips in turn.
-template <bool VEC, bool RADF>
-__global__ __launch_bounds__(1024) void k_adj_gathert(DevTables T, const double* __restrict__ gxs,
-                                                      const double* __restrict__ gys, const double* __restrict__ gxv,
-                                                      const double* __restrict__ gyv, double* __restrict__ Y,
-                                                      double* __restrict__ Wn, double beta, int sfield,
-                                                      long long field_stride_in, int pass) {
-  extern __shared__ double sg[];  // [3 layers][n]: scalar; then theta, phi parts for vectors
-  const int n = T.n, nr = T.nr;
+// (sender panel s, sender edge se, chunk of planes).  Per plane the
+// receiver's ghost values of that strip (receiver from the inverse route)
+// are first staged in LDS in the sender's along-edge order -- for vectors
+// already through the receiver-basis transform, as theta and phi parts --
+// then each strip point (layer l, along index a) sums the receiver stencils
+// that read it (the transposed rows gt_*), applies the sender's
+// (xi,eta)->(theta,phi) conversion transposed for vectors, and adds the
+// result to Y (and beta times it to Wn).  pass 0 covers the S/N strips, pass
+// 1 the W/E strips: strips of one pass never share a point, so the adds need
+// no atomics; corner points get their two strips in turn.  The transposed
+// rows, transforms and conversions do not depend on the plane: each thread
+// holds its own in registers.  RADF applies FILTER_R^T (filter.f90:682-693)
+// to the per-plane ghost adjoints from an 11-plane register window, one new
+// plane per step.
+// Threads per block: vector strips (whose per-point tables take ~70
+// registers) at most 512, scalar strips as the forward gather.
+constexpr int gathert_threads_max(bool vec, int ni) { return vec ? 512 : gather_threads_max(ni); }
+
+template <bool VEC, bool RADF, int NI>
+__global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
+    DevTables T, const double* __restrict__ gxs, const double* __restrict__ gys, const double* __restrict__ gxv,
+    const double* __restrict__ gyv, double* __restrict__ Y, double* __restrict__ Wn, double beta, int sfield,
+    long long field_stride_in, int pass) {
+  // 2 buffers x [3 layers][n]: scalar; then theta, phi parts for vectors
+  extern __shared__ double sg[];
+  constexpr int NQ = VEC ? 3 : 1;
+  const int n = T.n, nr = T.nr, strip = 3 * n, bstride = NQ * strip;
+  const int kc = gather_kc(nr);
   const int b = blockIdx.x;
-  const int k = b / 12;
+  const int k0 = (b / 12) * kc, k1 = min(k0 + kc, nr);
   const int s = (b % 12) / 2;
   const int se = (pass == 0 ? kS : kW) + (b & 1);
   const int fld = blockIdx.y;
@@ -447,69 +457,136 @@ __global__ __launch_bounds__(1024) void k_adj_gathert(DevTables T, const double*
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t fo = (size_t)fld * field_stride_in;
   const size_t kst = (size_t)3 * n;  // plane stride of both ghost layouts
-  // phase 1: ghosts (P, e, k, m, l) in the receiver layout, read in memory order
   const double* gsrc = (e < 2) ? gxs : gys;
   const double* vsrc = (e < 2) ? gxv : gyv;
-  const size_t gbase = (e < 2) ? fo + (size_t)(P * 2 + e) * nr * kst : fo + (size_t)(P * 2 + (e - 2)) * nr * kst;
-  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    // x ghosts [m][l] (l fastest), y ghosts [l][m]
-    const int l = (e < 2) ? t % 3 + 1 : t / n + 1;
-    const int m = (e < 2) ? t / 3 : t % n;
-    const int mp = rev ? n - 1 - m : m;
-    double gs;
-    if (RADF) {  // FILTER_R^T over the per-plane ghost adjoints (filter.f90:682-693)
-      gs = 0.0;
-      for (int kk = max(k - kNstr, kNstr); kk <= min(k + kNstr, nr - 1 - kNstr); ++kk)
-        gs += T.filtr[k - kk + 5] * gsrc[gbase + (size_t)kk * kst + t];
-    } else {
-      gs = gsrc[gbase + (size_t)k * kst + t];
-    }
-    sg[(l - 1) * n + mp] = gs;
-    if (VEC) {
-      const double gv = vsrc[gbase + (size_t)k * kst + t];
-      const double* ab = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
-      sg[3 * n + (l - 1) * n + mp] = ab[0] * gv;
-      sg[6 * n + (l - 1) * n + mp] = ab[1] * gv;
-    }
-  }
-  __syncthreads();
-  // phase 2: strip points, in the sender's memory order
+  const size_t gside = fo + (size_t)(P * 2 + (e < 2 ? e : e - 2)) * nr * kst;
+  const bool col = (se == kW || se == kE);  // column strips: layer index fastest in memory
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
-  double* ys = Y + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane;
-  double* ws = Wn ? Wn + ((size_t)(s * 5 + sfield + fld) * nr + k) * plane : nullptr;
-  double* y1 = Y + ((size_t)(s * 5 + 1) * nr + k) * plane;
-  double* y2 = y1 + nf;
-  double* w1 = Wn ? Wn + ((size_t)(s * 5 + 1) * nr + k) * plane : nullptr;
-  double* w2 = Wn ? w1 + nf : nullptr;
-  const bool col = (se == kW || se == kE);  // column strips: layer index fastest in memory
-  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    const int l = col ? t % 3 + 1 : t / n + 1;
-    const int a = col ? t / 3 : t % n;
-    const size_t r0 = ((size_t)(l - 1) * n + a) * kGtWidth;
-    double as = 0.0, ath = 0.0, aph = 0.0;
+
+  // this thread's receiver ghosts (staging) and strip points (sums)
+  bool val[NI];
+  int slot[NI], gmp[NI][kGtWidth], idx[NI];
+  size_t gofs[NI];
+  double ab[NI][2], gw[NI][kGtWidth], c[NI][6];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int t = threadIdx.x + it * blockDim.x;
+    val[it] = t < strip;
+    const int tt = val[it] ? t : 0;
+    // x ghosts [m][l] (l fastest), y ghosts [l][m]
+    const int l = (e < 2) ? tt % 3 + 1 : tt / n + 1;
+    const int m = (e < 2) ? tt / 3 : tt % n;
+    const int mp = rev ? n - 1 - m : m;
+    slot[it] = (l - 1) * n + mp;
+    gofs[it] = gside + tt;
+    if (VEC) {
+      const double* abp = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
+      ab[it][0] = abp[0];
+      ab[it][1] = abp[1];
+    }
+    const int ls = col ? tt % 3 + 1 : tt / n + 1;
+    const int a = col ? tt / 3 : tt % n;
+    const size_t r0 = ((size_t)(ls - 1) * n + a) * kGtWidth;
 #pragma unroll
     for (int q = 0; q < kGtWidth; ++q) {
-      const int mp = T.gt_mp[r0 + q];
-      const double w = T.gt_w[r0 + q];
-      as += w * sg[(l - 1) * n + mp];
+      gmp[it][q] = (ls - 1) * n + T.gt_mp[r0 + q];
+      gw[it][q] = T.gt_w[r0 + q];
+    }
+    idx[it] = strip_index_a(se, ls, a, n);
+    if (VEC) {
+#pragma unroll
+      for (int x = 0; x < 6; ++x) c[it][x] = cv[(size_t)idx[it] * 6 + x];
+    }
+  }
+  double* ys = Y + (size_t)(s * 5 + sfield + fld) * nf;
+  double* ws = Wn ? Wn + (size_t)(s * 5 + sfield + fld) * nf : nullptr;
+  double* y1 = Y + (size_t)(s * 5 + 1) * nf;
+  double* y2 = y1 + nf;
+  double* w1 = Wn ? Wn + (size_t)(s * 5 + 1) * nf : nullptr;
+  double* w2 = Wn ? w1 + nf : nullptr;
+
+  // RADF: window of per-plane ghost adjoints over planes k-5..k+5 (zero
+  // outside the filtered planes [5, nr-6], which FILTER_R leaves alone)
+  double win[NI][11];
+  auto gat = [&](int it, int kk) {
+    return (kk >= kNstr && kk <= nr - 1 - kNstr) ? gsrc[gofs[it] + (size_t)kk * kst] : 0.0;
+  };
+  if (RADF) {
+#pragma unroll
+    for (int it = 0; it < NI; ++it)
+#pragma unroll
+      for (int q = 0; q < 10; ++q) win[it][q + 1] = gat(it, k0 - 5 + q);
+  }
+  for (int k = k0; k < k1; ++k) {
+    double* buf = sg + ((k - k0) & 1) * bstride;
+    const size_t kp = (size_t)k * plane;
+    // read-modify-write inputs of this plane, issued before the barrier
+    double yv[NI], wv[NI], y1v[NI], y2v[NI], w1v[NI], w2v[NI];
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!val[it]) continue;
+      yv[it] = ys[kp + idx[it]];
+      if (ws) wv[it] = ws[kp + idx[it]];
       if (VEC) {
-        ath += w * sg[3 * n + (l - 1) * n + mp];
-        aph += w * sg[6 * n + (l - 1) * n + mp];
+        y1v[it] = y1[kp + idx[it]];
+        y2v[it] = y2[kp + idx[it]];
+        if (w1) {
+          w1v[it] = w1[kp + idx[it]];
+          w2v[it] = w2[kp + idx[it]];
+        }
       }
     }
-    const int idx = strip_index_a(se, l, a, n);
-    ys[idx] += as;
-    if (ws) ws[idx] += beta * as;
-    if (VEC) {
-      const double* c = cv + (size_t)idx * 6;
-      const double ax = c[0] * c[1] * ath + c[3] * c[4] * aph;
-      const double ay = c[0] * c[2] * ath + c[3] * c[5] * aph;
-      y1[idx] += ax;
-      y2[idx] += ay;
-      if (w1) {
-        w1[idx] += beta * ax;
-        w2[idx] += beta * ay;
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!val[it]) continue;
+      double gsv;
+      if (RADF) {
+#pragma unroll
+        for (int q = 0; q < 10; ++q) win[it][q] = win[it][q + 1];
+        win[it][10] = gat(it, k + 5);
+        gsv = 0.0;  // sum over kk = k-5..k+5 ascending: filtr[k - kk + 5] = filtr[10 - q]
+#pragma unroll
+        for (int q = 0; q < 11; ++q) {
+          const int kk = k - 5 + q;
+          if (kk >= kNstr && kk <= nr - 1 - kNstr) gsv += T.filtr[10 - q] * win[it][q];
+        }
+      } else {
+        gsv = gsrc[gofs[it] + (size_t)k * kst];
+      }
+      buf[slot[it]] = gsv;
+      if (VEC) {
+        const double gv = vsrc[gofs[it] + (size_t)k * kst];
+        buf[strip + slot[it]] = ab[it][0] * gv;
+        buf[2 * strip + slot[it]] = ab[it][1] * gv;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!val[it]) continue;
+      double as = 0.0, ath = 0.0, aph = 0.0;
+#pragma unroll
+      for (int q = 0; q < kGtWidth; ++q) {
+        const double w = gw[it][q];
+        as += w * buf[gmp[it][q]];
+        if (VEC) {
+          ath += w * buf[strip + gmp[it][q]];
+          aph += w * buf[2 * strip + gmp[it][q]];
+        }
+      }
+      const size_t x = kp + idx[it];
+      ys[x] = yv[it] + as;
+      if (ws) ws[x] = wv[it] + beta * as;
+      if (VEC) {
+        const double ax = c[it][0] * c[it][1] * ath + c[it][3] * c[it][4] * aph;
+        const double ay = c[it][0] * c[it][2] * ath + c[it][3] * c[it][5] * aph;
+        y1[x] = y1v[it] + ax;
+        y2[x] = y2v[it] + ay;
+        if (w1) {
+          w1[x] = w1v[it] + beta * ax;
+          w2[x] = w2v[it] + beta * ay;
+        }
       }
     }
   }
@@ -713,27 +790,43 @@ hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream
   return d32 ? launch_adj_stage_t<16, 16, true>(T, a, st) : launch_adj_stage_t<16, 16, false>(T, a, st);
 }
 
-hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
-                              const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
-                              hipStream_t st, bool radial_filter) {
+template <bool VEC, bool RADF, int NI>
+static hipError_t launch_gathert(const DevTables& T, const double* gxs, const double* gys, const double* gxv,
+                                 const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
+                                 hipStream_t st) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
-  dim3 grid((unsigned)(12 * T.nr), (unsigned)nfields);
-  const int nt = std::min(1024, (3 * T.n + 63) / 64 * 64);
-  const size_t lds = sizeof(double) * (vec ? 9 : 3) * (size_t)T.n;
+  dim3 grid((unsigned)(12 * gather_blocks_per_edge(T.nr)), (unsigned)nfields);
+  const int nt = VEC ? std::min(512, ((3 * T.n + NI - 1) / NI + 63) / 64 * 64) : gather_threads(T.n, NI);
+  const size_t lds = sizeof(double) * 2 * (VEC ? 9 : 3) * (size_t)T.n;  // n <= 512: at most 72 KiB
+  if (lds > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_gathert<VEC, RADF, NI>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
   for (int pass = 0; pass < 2; ++pass) {
-    if (vec)
-      hipLaunchKernelGGL((k_adj_gathert<true, false>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                         sfield, field_stride, pass);
-    else if (radial_filter)
-      hipLaunchKernelGGL((k_adj_gathert<false, true>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                         sfield, field_stride, pass);
-    else
-      hipLaunchKernelGGL((k_adj_gathert<false, false>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                         sfield, field_stride, pass);
+    hipLaunchKernelGGL((k_adj_gathert<VEC, RADF, NI>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
+                       sfield, field_stride, pass);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
+                              const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
+                              hipStream_t st, bool radial_filter) {
+  if (vec) {  // strip points per thread: 1 (n <= 170), 2 (n <= 341), 3 (n <= 512) at 512 threads
+    const int ni = (3 * T.n + 511) / 512;
+    if (ni == 1) return launch_gathert<true, false, 1>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+    if (ni == 2) return launch_gathert<true, false, 2>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+    return launch_gathert<true, false, 3>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+  }
+  const bool two = 3 * T.n > 1024;  // n > 341: two strip points per thread
+  if (radial_filter)
+    return two ? launch_gathert<false, true, 2>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st)
+               : launch_gathert<false, true, 1>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+  return two ? launch_gathert<false, false, 2>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st)
+             : launch_gathert<false, false, 1>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
 }
 
 template <int TI, int TJ>

@@ -171,11 +171,6 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
 
 // blockIdx.x = (chunk of planes)*24 + (receiver panel P)*4 + (receiver edge e);
 // blockIdx.y selects a field offset (the filter's five scalars).
-// Threads per gather block: one strip point each up to 1024 (n <= 341), else
-// two each (n <= 512).
-constexpr int gather_threads_max(int ni) { return ni == 1 ? 1024 : 768; }
-inline int gather_threads(int n, int ni) { return (((3 * n + ni - 1) / ni) + 63) / 64 * 64; }
-
 template <bool VEC, typename S, int NI>
 __global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
                                                  double* __restrict__ gxs, double* __restrict__ gys,

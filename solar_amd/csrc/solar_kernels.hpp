@@ -79,6 +79,10 @@ hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, in
 // blocks per receiver edge.
 constexpr int gather_kc(int nr) { return (nr + 31) / 32; }
 constexpr int gather_blocks_per_edge(int nr) { return (nr + gather_kc(nr) - 1) / gather_kc(nr); }
+// Threads per gather block: one strip point each up to 1024 (n <= 341), else
+// two each (n <= 512).
+constexpr int gather_threads_max(int ni) { return ni == 1 ? 1024 : 768; }
+inline int gather_threads(int n, int ni) { return (((3 * n + ni - 1) / ni) + 63) / 64 * 64; }
 // mode 0: count of non-finite values -> acc[0]; mode 1: sum of squares of
 // `field` over all panels -> acc[1].
 hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st, int field = 3);

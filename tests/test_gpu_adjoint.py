@@ -70,6 +70,23 @@ def test_filter_transpose(n, nr, kind):
     assert gap < 1e-13
 
 
+@pytest.mark.parametrize("n,nr", [(176, 13), (352, 12)])
+def test_transposes_wide_panels(n, nr):
+    """L and F transposes where the transposed gathers hold 2 (n = 176) and 3
+    (n = 352) strip points per thread."""
+    cfg, bg, _ = _setup(n, nr, 1)
+    rng = np.random.default_rng(n + nr)
+    x = rng.standard_normal((6, 5, nr, n, n))
+    w = rng.standard_normal((6, 5, nr, n, n))
+    with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        gap, a, b = _transpose_gap(s.compute_rhs(x, RHS_BC | RHS_SPONGE), w, x, s.adj_rhs(w))
+        print(f"[adj] L n={n} nr={nr}: gap={gap:.2e}", flush=True)
+        assert gap < 1e-13
+        gap, a, b = _transpose_gap(s.filter_exp(x), w, x, s.adj_filter(w))
+        print(f"[adj] F n={n} nr={nr}: gap={gap:.2e}", flush=True)
+        assert gap < 1e-13
+
+
 @pytest.mark.parametrize("n,nr,kind", CASES)
 def test_step_transpose(n, nr, kind):
     cfg, bg, dt = _setup(n, nr, kind)
