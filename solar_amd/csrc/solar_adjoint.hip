@@ -609,48 +609,88 @@ __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
 // (FILTER_R reads no other plane's output).  The eta ghosts come from bbar
 // itself (FILTER_ETA reads unfiltered ghosts, filter.f90:43), the xi ghosts
 // from abar = FILTER_ETA^T bbar on the edge columns.  One block per (panel,
-// field, edge, plane), one thread per (layer, along index).
-__global__ __launch_bounds__(256) void k_adj_filter_ghosts(DevTables T, const double* __restrict__ bin,
-                                                           double* __restrict__ fgxa, double* __restrict__ fgya) {
-  const int n = T.n, nr = T.nr;
-  int b = blockIdx.x;
-  const int kk = b % nr;
-  b /= nr;
-  const int e = b % 4;
-  const int PF = b / 4;
+// field, edge, chunk of planes): per plane the three edge rows (S/N) or
+// columns (W/E) of bbar go to LDS, one value per thread, with the next
+// plane's value loading meanwhile; each thread then forms one (layer, along
+// index) output.
+template <int NI>
+__global__ __launch_bounds__(gather_threads_max(NI)) void k_adj_filter_ghosts(DevTables T,
+                                                                               const double* __restrict__ bin,
+                                                                               double* __restrict__ fgxa,
+                                                                               double* __restrict__ fgya) {
+  extern __shared__ double sl[];  // 2 buffers x 3 lines x n
+  const int n = T.n, nr = T.nr, strip = 3 * n;
+  const int kc = gather_kc(nr);
+  const int b = blockIdx.x;
+  const int k0 = (b / 120) * kc, k1 = min(k0 + kc, nr);
+  const int e = (b % 120) % 4, PF = (b % 120) / 4;
   const int P = PF / 5, F = PF % 5;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const double* g = bin + (size_t)PF * nf + (size_t)kk * plane;
+  const double* g = bin + (size_t)PF * nf;
   const size_t gx_side = (size_t)nr * n * 3, gy_side = (size_t)nr * 3 * n;
   const size_t fst = (size_t)12 * gx_side;
-  const bool live = kk >= kNstr && kk <= nr - 1 - kNstr;
-  for (int t = threadIdx.x; t < 3 * n; t += blockDim.x) {
-    if (e >= 2) {  // S / N: rows 0..2 or n-3..n-1 of bbar, along i
-      const int l = t / n + 1, i = t % n;
+  const bool colw = e < 2;                       // W/E: three edge columns, all rows
+  const int c0 = (e == 1) ? n - 3 : 0, r0 = (e == 3) ? n - 3 : 0;
+  int src[NI], slot[NI];
+  bool val[NI];
+  size_t ob[NI];
+  int ol[NI], om[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int t = threadIdx.x + it * blockDim.x;
+    val[it] = t < strip;
+    const int tt = val[it] ? t : 0;
+    if (colw) {  // row tt/3, column c0 + tt%3 -> line tt%3, along index tt/3
+      src[it] = (tt / 3) * n + c0 + tt % 3;
+      slot[it] = (tt % 3) * n + tt / 3;
+      ol[it] = tt % 3 + 1;
+      om[it] = tt / 3;
+      ob[it] = F * fst + (size_t)(P * 2 + e) * gx_side + (size_t)om[it] * 3 + (ol[it] - 1);  // + k * 3n
+    } else {     // row r0 + tt/n, column tt%n -> line tt/n, along index tt%n
+      src[it] = (r0 + tt / n) * n + tt % n;
+      slot[it] = tt;
+      ol[it] = tt / n + 1;
+      om[it] = tt % n;
+      ob[it] = F * fst + (size_t)(P * 2 + (e - 2)) * gy_side + (size_t)(ol[it] - 1) * n + om[it];  // + k * 3n
+    }
+  }
+  double nxt[NI];
+  auto load = [&](int k) {
+#pragma unroll
+    for (int it = 0; it < NI; ++it) nxt[it] = g[(size_t)k * plane + src[it]];
+  };
+  load(k0);
+  for (int k = k0; k < k1; ++k) {
+    double* ln = sl + ((k - k0) & 1) * strip;
+#pragma unroll
+    for (int it = 0; it < NI; ++it)
+      if (val[it]) ln[slot[it]] = nxt[it];
+    load(min(k + 1, k1 - 1));  // past the last plane: loaded and not used
+    __syncthreads();
+    const bool live = k >= kNstr && k <= nr - 1 - kNstr;
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      if (!val[it]) continue;
+      const int l = ol[it], m = om[it];
       double s = 0.0;
       if (live) {
-        if (e == 2) {
-          for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * g[(size_t)jj * n + i];
-        } else {
-          for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * g[(size_t)jj * n + i];
-        }
-      }
-      fgya[F * fst + (size_t)(P * 2 + (e - 2)) * gy_side + ((size_t)kk * 3 + (l - 1)) * n + i] = s;
-    } else {  // W / E: abar on columns 0..2 or n-3..n-1, along j (layer index fastest)
-      const int l = t % 3 + 1, j = t / 3;
-      double s = 0.0;
-      if (live) {
-        const int ia = (e == 0) ? 0 : n - 4 + l, ib = (e == 0) ? 3 - l : n - 1;
-        for (int ii = ia; ii <= ib; ++ii) {
-          double a = 0.0;  // abar(ii, j) = sum_q f[6-q] bbar(ii, j-3+q), zero beyond the panel
-          for (int q = 0; q < 7; ++q) {
-            const int jj = j - 3 + q;
-            if (jj >= 0 && jj < n) a += T.filt[6 - q] * g[(size_t)jj * n + ii];
+        if (e == 2) {  // rows 0..2 at column m
+          for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * ln[jj * n + m];
+        } else if (e == 3) {  // rows n-3..n-1 at column m
+          for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * ln[(jj - r0) * n + m];
+        } else {  // abar on the edge columns at row m
+          const int ia = (e == 0) ? 0 : n - 4 + l, ib = (e == 0) ? 3 - l : n - 1;
+          for (int ii = ia; ii <= ib; ++ii) {
+            double av = 0.0;  // abar(ii, m) = sum_q f[6-q] bbar(ii, m-3+q), zero beyond the panel
+            for (int q = 0; q < 7; ++q) {
+              const int jj = m - 3 + q;
+              if (jj >= 0 && jj < n) av += T.filt[6 - q] * ln[(ii - c0) * n + jj];
+            }
+            s += ((e == 0) ? T.filt[3 - l - ii] : T.filt[n + 2 + l - ii]) * av;
           }
-          s += ((e == 0) ? T.filt[3 - l - ii] : T.filt[n + 2 + l - ii]) * a;
         }
       }
-      fgxa[F * fst + (size_t)(P * 2 + e) * gx_side + ((size_t)kk * n + j) * 3 + (l - 1)] = s;
+      (colw ? fgxa : fgya)[ob[it] + (size_t)k * 3 * n] = s;
     }
   }
 }
@@ -839,7 +879,12 @@ static hipError_t launch_adj_filter_t(const DevTables& T, const double* in, doub
   if (nkch > maxk) nkch = maxk;
   const int kchunk = (T.nr + nkch - 1) / nkch;
   hipLaunchKernelGGL((k_adj_filter<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, in, out, kchunk, nkch);
-  hipLaunchKernelGGL(k_adj_filter_ghosts, dim3(30 * 4 * T.nr), dim3(256), 0, st, T, in, fgxa, fgya);
+  const size_t lds = sizeof(double) * 2 * 3 * (size_t)T.n;
+  const dim3 grid((unsigned)(120 * gather_blocks_per_edge(T.nr)));
+  if (3 * T.n > 1024)
+    hipLaunchKernelGGL(k_adj_filter_ghosts<2>, grid, dim3(gather_threads(T.n, 2)), lds, st, T, in, fgxa, fgya);
+  else
+    hipLaunchKernelGGL(k_adj_filter_ghosts<1>, grid, dim3(gather_threads(T.n, 1)), lds, st, T, in, fgxa, fgya);
   return hipGetLastError();
 }
 
