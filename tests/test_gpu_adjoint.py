@@ -70,10 +70,11 @@ def test_filter_transpose(n, nr, kind):
     assert gap < 1e-13
 
 
-@pytest.mark.parametrize("n,nr", [(176, 13), (352, 12)])
+@pytest.mark.parametrize("n,nr", [(176, 13), (352, 12), (512, 12)])
 def test_transposes_wide_panels(n, nr):
     """L and F transposes where the transposed gathers hold 2 (n = 176) and 3
-    (n = 352) strip points per thread."""
+    (n = 352, 512) strip points per thread; at n = 512 the staged lines take
+    more than 64 KiB of LDS."""
     cfg, bg, _ = _setup(n, nr, 1)
     rng = np.random.default_rng(n + nr)
     x = rng.standard_normal((6, 5, nr, n, n))

@@ -194,10 +194,11 @@ def test_rk4_tile_shapes(n, nr, kind, steps):
         assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
 
 
-@pytest.mark.parametrize("n,nr", [(352, 13), (64, 45)])
+@pytest.mark.parametrize("n,nr", [(352, 13), (512, 12), (64, 45)])
 def test_ghosts_plane_chunks(n, nr):
-    """Ghost gathers with two strip points per thread (3n > 1024) and with a
-    ragged last chunk of planes (nr not a multiple of the planes per block)."""
+    """Ghost gathers with two strip points per thread (3n > 1024; at n = 512
+    the staged lines take more than 64 KiB of LDS) and with a ragged last
+    chunk of planes (nr not a multiple of the planes per block)."""
     from solar_amd import background
     bg = background(1, nr)
     print(f"[case] ghosts n={n} nr={nr}", flush=True)
