@@ -244,9 +244,10 @@ int solar_ens_finalize(solar_ens* ens);
 
 /* Times the candidate tile shapes and radial chunk lengths of the stage and
  * filter kernels for every class of members with equal (n, nr), reps
- * launches each, keeps the fastest, then picks one block per work item or a
- * persistent grid from the time of a whole step.  Clears every state: call it
- * before solar_ens_set_state. */
+ * launches each, and keeps the fastest; keeps those choices only if a whole
+ * step with them beats one with the default shapes; then picks one block per
+ * work item or a persistent grid from the time of a whole step.  Clears
+ * every state: call it before solar_ens_set_state. */
 int solar_ens_autotune(solar_ens* ens, int reps);
 
 /* Launch parameters of member m: six[] = stage TI, TJ, planes per chunk,
