@@ -133,13 +133,18 @@ def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
     return out
 
 
-def _c3_bench(steps, cpu):
+def _workload_descr(name, bits):
+    d = DESCR[name]
+    return d if bits == 64 else d.replace("FP64", "FP32 state, FP64 arithmetic")
+
+
+def _c3_bench(steps, cpu, bits=64):
     """The 6x256x256x384 grid: GPU rate, stage-kernel roofline, and a one-step
     CPU sample of the same grid (after a warm-up step)."""
     from solar_amd import CONFIGS, Solver, make_inputs
     cfg = CONFIGS["c3"]
     bg, q0, dt, _ = make_inputs(cfg)
-    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits) as s:
         s.set_state(q0)
         s.rk4(5, dt)
         s.sync()
@@ -149,7 +154,7 @@ def _c3_bench(steps, cpu):
         wall = time.perf_counter() - t
         ok = s.check_finite()
     os.environ["SOLAR_PROFILE"] = "1"
-    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    p = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits)
     os.environ.pop("SOLAR_PROFILE")
     p.set_state(q0)
     p.rk4(2, dt)
@@ -157,9 +162,10 @@ def _c3_bench(steps, cpu):
     kt = p.kernel_times()
     p.close()
     rate = cfg.points * steps / wall
-    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points
-    out = {"workload": DESCR["c3"], "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
-           "ms_per_step": wall / steps * 1e3, "step_frac": rate * STEP_BYTES / HBM_PEAK,
+    scale = bits / 64.0
+    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points * scale
+    out = {"workload": _workload_descr("c3", bits), "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
+           "ms_per_step": wall / steps * 1e3, "step_frac": rate * STEP_BYTES * scale / HBM_PEAK,
            "stage_frac": stage_bytes / (kt["stage_ms"] * 1e-3) / HBM_PEAK, "state_finite": ok,
            "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
     if cpu:
@@ -285,7 +291,9 @@ def main():
     stage_s = kt["stage_ms"] * 1e-3
     achieved = stage_bytes / stage_s
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_stage_{cfg.name}.json")
+    # counter-measured HBM bytes of the stage kernel, for this workload and storage width
+    suffix = "" if bits == 64 else "_fp32state"
+    tpath = os.path.join(ROOT, "profiles", f"traffic_stage_{cfg.name}{suffix}.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -311,8 +319,8 @@ def main():
         del fin
 
     c3 = None
-    if not args.no_c3 and bits == 64 and args.workload != "c3":
-        c3 = _c3_bench(200, not args.no_cpu)
+    if not args.no_c3 and args.workload != "c3":
+        c3 = _c3_bench(200, not args.no_cpu and bits == 64, bits)
 
     ensemble = None
     if not args.no_ensemble and bits == 64:
@@ -329,7 +337,7 @@ def main():
     out = {"metric": METRIC, "value": rate, "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
            "warmup": args.warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64" if bits == 64 else "f32-state/f64-arith", "data": "synthetic",
-           "config": {"workload": DESCR[cfg.name], "n": cfg.n, "nr": cfg.nr, "panels": 6, "points": cfg.points,
+           "config": {"workload": _workload_descr(cfg.name, bits), "n": cfg.n, "nr": cfg.nr, "panels": 6, "points": cfg.points,
                       "dt": dt, "seed": cfg.seed, "state_finite": ok},
            "roofline": roofline, "cpu_baseline": cpu}
     if precision:
