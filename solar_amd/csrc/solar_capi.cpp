@@ -55,6 +55,7 @@ struct solar_ctx {
   void* qw = nullptr;               // 2N accumulator, same type
   double *gxp = nullptr, *gyp = nullptr, *gxv = nullptr, *gyv = nullptr;
   double *fgx = nullptr, *fgy = nullptr;
+  void* ecols = nullptr;  // edge-column copies written by the stages for the next gather (state type)
   double* acc = nullptr;
   LaunchShape shape;
   // hipGraph of one RK4 step for a given dt
@@ -120,19 +121,22 @@ int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = null
   void* nxt = b1 ? b1 : c->q[1];
   void* qw = bw ? bw : c->qw;
   int e = 0;
+  // Every stage also writes its output's edge columns to c->ecols, from which
+  // the next gather reads its W/E strips; the first gather reads the state.
   for (int s = 0; s < 5; ++s) {
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-    HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream));
+    HIPCHK(solar::launch_gather(T, cur, c->f32, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream,
+                                s > 0 ? c->ecols : nullptr));
     if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
     HIPCHK(solar::launch_stage(T, cur, nxt, qw, c->f32, c->gxp, c->gyp, c->gxv, c->gyv, s, dt, 0, c->stream,
                                c->shape, c->c2f, divp ? divp[s] : nullptr, c->div_bits == 32, tl_dc2,
-                               tl_dc2 ? tl_divp[s] : nullptr));
+                               tl_dc2 ? tl_divp[s] : nullptr, c->ecols));
     void* t = cur;
     cur = nxt;
     nxt = t;
   }
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
-  HIPCHK(solar::launch_gather(T, cur, c->f32, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
+  HIPCHK(solar::launch_gather(T, cur, c->f32, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream, c->ecols));
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
   HIPCHK(solar::launch_filter(T, cur, nxt, c->f32, c->fgx, c->fgy, c->stream, c->shape));
   if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
@@ -271,6 +275,11 @@ int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double
   c->q[0] = q0;
   c->q[1] = q1;
   c->qw = qwb;
+  if (!env_flag("SOLAR_NO_EDGE_COLS")) {
+    char* ec = nullptr;
+    if ((rc = dev_alloc(c, &ec, solar::edge_cols_elems(n, nr) * c->esz))) return bail(rc);
+    c->ecols = ec;
+  }
   if (hipMemset(c->q[0], 0, nq * c->esz) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipMemset failed"));
   c->use_graph = !env_flag("SOLAR_NO_GRAPH");
   c->profile = env_flag("SOLAR_PROFILE");

@@ -57,24 +57,33 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
 // values load while this plane is interpolated, and the staged lines are
 // double-buffered in LDS, so one barrier per plane suffices.
 //   gx*: [P][side W/E][k][m][l],  gy*: [P][side S/N][k][l][m]
+//   ecols: when not null, the compact copies of the sender's edge columns
+//     (edge_cols layout, written by the stage that produced q); W/E strips
+//     are then read from there, one contiguous run per field and plane,
+//     instead of three values from every row of the field.
 //   NI: strip points per thread (3n <= NI * blockDim.x)
 //   sline: 2 buffers x 3 layers x (3 quantities for vectors, else 1) x n
 template <bool VEC, typename S, int NI>
 __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, int k0, int k1, int fld,
-                                            const S* __restrict__ q, int sfield, double* __restrict__ gxs,
-                                            double* __restrict__ gys, double* __restrict__ gxv,
-                                            double* __restrict__ gyv, long long field_stride_out, double* sline) {
+                                            const S* __restrict__ q, const S* __restrict__ ecols, int sfield,
+                                            double* __restrict__ gxs, double* __restrict__ gys,
+                                            double* __restrict__ gxv, double* __restrict__ gyv,
+                                            long long field_stride_out, double* sline) {
   constexpr int NQ = VEC ? 3 : 1;
   const int n = T.n, nr = T.nr;
   const Route rt = T.route[P * 4 + e];
   const int s = rt.sender, se = rt.sender_edge;
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const auto sp = to_global(q + (size_t)(s * 5 + sfield + fld) * nf);
-  const auto vx = to_global(q + (size_t)(s * 5 + 1) * nf);
-  const auto ve = vx + nf;
+  const int colstrip = (se == kW || se == kE);  // column strips: layer index is the fast memory index
+  const bool from_cols = colstrip && ecols != nullptr;
+  // source of field f at plane k, strip offset x: src[f * fstr + k * kstr + x]
+  const S* src = from_cols ? ecols + (size_t)(s * 2 + (se == kE)) * nr * edge_cols_plane(n) : q + (size_t)s * 5 * nf;
+  const size_t fstr = from_cols ? (size_t)3 * n : nf, kstr = from_cols ? edge_cols_plane(n) : plane;
+  const auto sp = to_global(src + (size_t)(sfield + fld) * fstr);
+  const auto vx = to_global(src + fstr);
+  const auto ve = vx + fstr;
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
-  const int colstrip = (se == kW || se == kE);  // column strips: layer index is the fast memory index
   const int lstride = NQ * n, bstride = 3 * lstride;
   const size_t fo = (size_t)fld * field_stride_out;
 
@@ -94,11 +103,12 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
     const int tt = sval[it] ? t : 0;
     const int l = colstrip ? tt % 3 + 1 : tt / n + 1;
     const int a = colstrip ? tt / 3 : tt % n;
-    sidx[it] = strip_index(se, l, a, n);
+    const int pidx = strip_index(se, l, a, n);  // point in the sender's plane
+    sidx[it] = from_cols ? tt : pidx;            // edge_cols: (a, l) at a * 3 + l - 1
     sslot[it] = (l - 1) * lstride + a;
     if (VEC) {
 #pragma unroll
-      for (int x = 0; x < 6; ++x) c[it][x] = cv[(size_t)sidx[it] * 6 + x];
+      for (int x = 0; x < 6; ++x) c[it][x] = cv[(size_t)pidx * 6 + x];
     }
     const int lo = (e < 2) ? tt % 3 + 1 : tt / n + 1;  // output layout: l fastest for x ghosts
     const int m = (e < 2) ? tt / 3 : tt % n;
@@ -121,7 +131,7 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
   // source values of one plane, in the storage type until staged
   S pv[NI], xv[NI], yv[NI];
   auto load = [&](int k) {
-    const size_t kp = (size_t)k * plane;
+    const size_t kp = (size_t)k * kstr;
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
       pv[it] = sp[kp + sidx[it]];
@@ -172,15 +182,16 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
 // blockIdx.x = (chunk of planes)*24 + (receiver panel P)*4 + (receiver edge e);
 // blockIdx.y selects a field offset (the filter's five scalars).
 template <bool VEC, typename S, int NI>
-__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, const S* __restrict__ q, int sfield,
+__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, const S* __restrict__ q,
+                                                 const S* __restrict__ ecols, int sfield,
                                                  double* __restrict__ gxs, double* __restrict__ gys,
                                                  double* __restrict__ gxv, double* __restrict__ gyv,
                                                  long long field_stride_out) {
   extern __shared__ double sline[];
   const int kc = gather_kc(T.nr);
   const int k0 = (blockIdx.x / 24) * kc, edge = blockIdx.x % 24;
-  gather_edge<VEC, S, NI>(T, edge / 4, edge % 4, k0, min(k0 + kc, T.nr), blockIdx.y, q, sfield, gxs, gys, gxv, gyv,
-                          field_stride_out, sline);
+  gather_edge<VEC, S, NI>(T, edge / 4, edge % 4, k0, min(k0 + kc, T.nr), blockIdx.y, q, ecols, sfield, gxs, gys,
+                          gxv, gyv, field_stride_out, sline);
 }
 
 // Values one thread loads for one radial plane of the stage kernel: its own
@@ -229,6 +240,10 @@ struct StagePlane {
 //   dt come from the member (the state ping-pongs between M.q[cur] and
 //   M.q[1-cur]); the single-grid arguments are unused.
 //   MODE 2: tangent-linear stage (tl_dc2, tl_div used).
+//   ecols_a (single grid, not null): the stage also stores its output at
+//   the edge columns i = 1..3 and n-4..n-2 in the edge_cols layout, for the
+//   next gather -- p', v_xi, v_eta after stages 1-4 (the vector gather), all
+//   five fields after the last stage (the filter gather).
 template <int TI, int TJ, int KIND, typename S, int MODE = 0>
 __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Targ, const S* __restrict__ qin_a,
                                                    S* __restrict__ qout_a, S* __restrict__ qw_a,
@@ -238,7 +253,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
                                                    int rhs_mode, const double* __restrict__ c2f,
                                                    double* __restrict__ divout, int div32,
                                                    const double* __restrict__ tl_dc2, const double* __restrict__ tl_div,
-                                                   const EnsMember* __restrict__ M,
+                                                   S* __restrict__ ecols_a, const EnsMember* __restrict__ M,
                                                    const int4* __restrict__ items, int nitems, int cur) {
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
@@ -293,6 +308,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const auto f4 = f3 + nf;                     // p'
   const auto o0 = qout + (size_t)(P * 5) * nf;
   const auto w0 = qw + (size_t)(P * 5) * nf;
+  // edge-column copy: layer l = i (W) or n-1-i (E); the others store none
+  const int ecl = i <= 3 ? i : n - 1 - i;
+  const bool ec_on = !ENS && !rhs_mode && ecols_a != nullptr && ecl >= 1 && ecl <= 3;
+  const auto ecw = to_global(ec_on ? ecols_a : qout_a);  // qout_a: never stored through when ec_on is false
+  const size_t ec_base = ec_on ? (size_t)(P * 2 + (i > 3)) * nr * edge_cols_plane(n) + (size_t)j * 3 + (ecl - 1) : 0;
 
   // Halo cells this thread fills: source pointer and plane stride, from the
   // field inside the panel or from the gathered ghost strips beyond its edge.
@@ -424,6 +444,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   load_plane(k0, nxt);
   S pq0, pq1, pq2, pq3, pq4, pw0, pw1, pw2, pw3, pw4;  // the previous plane's q, qw (or RHS)
   size_t px = 0;
+  int pk = 0;  // the plane of px
   auto flush = [&]() {
     const S pq[5] = {pq0, pq1, pq2, pq3, pq4}, pwv[5] = {pw0, pw1, pw2, pw3, pw4};
     if (rhs_mode) {
@@ -434,6 +455,12 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
       for (int f = 0; f < 5; ++f) {
         __builtin_nontemporal_store(pq[f], o0 + f * nf + px);
         if (KIND != 2) __builtin_nontemporal_store(pwv[f], w0 + f * nf + px);
+      }
+      if (ec_on) {
+        const size_t x = ec_base + (size_t)pk * edge_cols_plane(n);
+#pragma unroll
+        for (int f = 0; f < 5; ++f)
+          if (KIND == 2 || f == 1 || f == 2 || f == 4) ecw[x + (size_t)f * 3 * n] = pq[f];
       }
     }
   };
@@ -538,6 +565,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
 
     px = (size_t)k * plane + ij;
+    pk = k;
     if (rhs_mode) {
       pq0 = static_cast<S>(r1);
       pq1 = static_cast<S>(r2);
@@ -817,11 +845,11 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_gather_ens(const Ens
   const int k0 = (li / 24) * kc, edge = li % 24, k1 = min(k0 + kc, nr);
   const long long fs = 12LL * nr * E.T.n * 3;
   if (VEC)
-    gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], 4, E.gxp, E.gyp, E.gxv,
-                                  E.gyv, fs, sline);
+    gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 4, E.gxp, E.gyp,
+                                  E.gxv, E.gyv, fs, sline);
   else
-    gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], 0, E.fgx, E.fgy, nullptr,
-                                   nullptr, fs, sline);
+    gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 0, E.fgx, E.fgy,
+                                   nullptr, nullptr, fs, sline);
 }
 
 // Non-finite scan and v_r sum of squares; acc[0] counts non-finite values,
@@ -881,8 +909,8 @@ static hipError_t gather_lds(K kernel, size_t lds) {
 }
 
 template <bool VEC, typename S, int NI>
-static hipError_t launch_gather_v(const DevTables& T, const S* q, int sfield, int nfields, double* gxs, double* gys,
-                                  double* gxv, double* gyv, hipStream_t st) {
+static hipError_t launch_gather_v(const DevTables& T, const S* q, const S* ecols, int sfield, int nfields,
+                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
   const int kc = gather_kc(T.nr);
   dim3 grid((unsigned)(24 * ((T.nr + kc - 1) / kc)), (unsigned)nfields);
@@ -890,32 +918,36 @@ static hipError_t launch_gather_v(const DevTables& T, const S* q, int sfield, in
   const size_t lds = sizeof(double) * 2 * 3 * (VEC ? 3 : 1) * (size_t)T.n;  // n <= 512: at most 72 KiB
   hipError_t e = gather_lds(&k_gather<VEC, S, NI>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_gather<VEC, S, NI>), grid, dim3(nt), lds, st, T, q, sfield, gxs, gys, gxv, gyv, field_stride);
+  hipLaunchKernelGGL((k_gather<VEC, S, NI>), grid, dim3(nt), lds, st, T, q, ecols, sfield, gxs, gys, gxv, gyv,
+                     field_stride);
   return hipGetLastError();
 }
 
 template <typename S>
-static hipError_t launch_gather_t(const DevTables& T, const S* q, bool vec, int sfield, int nfields, double* gxs,
-                                  double* gys, double* gxv, double* gyv, hipStream_t st) {
+static hipError_t launch_gather_t(const DevTables& T, const S* q, const S* ecols, bool vec, int sfield, int nfields,
+                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st) {
   const bool two = 3 * T.n > 1024;  // n > 341: two strip points per thread
   if (vec)
-    return two ? launch_gather_v<true, S, 2>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st)
-               : launch_gather_v<true, S, 1>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st);
-  return two ? launch_gather_v<false, S, 2>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st)
-             : launch_gather_v<false, S, 1>(T, q, sfield, nfields, gxs, gys, gxv, gyv, st);
+    return two ? launch_gather_v<true, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st)
+               : launch_gather_v<true, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st);
+  return two ? launch_gather_v<false, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st)
+             : launch_gather_v<false, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st);
 }
 
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
-                         double* gys, double* gxv, double* gyv, hipStream_t st) {
-  if (f32) return launch_gather_t(T, static_cast<const float*>(q), vec, sfield, nfields, gxs, gys, gxv, gyv, st);
-  return launch_gather_t(T, static_cast<const double*>(q), vec, sfield, nfields, gxs, gys, gxv, gyv, st);
+                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols) {
+  if (f32)
+    return launch_gather_t(T, static_cast<const float*>(q), static_cast<const float*>(ecols), vec, sfield, nfields,
+                           gxs, gys, gxv, gyv, st);
+  return launch_gather_t(T, static_cast<const double*>(q), static_cast<const double*>(ecols), vec, sfield, nfields,
+                         gxs, gys, gxv, gyv, st);
 }
 
 template <int TI, int TJ, typename S>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
-                                 double* divout, int div32, const double* tl_dc2, const double* tl_div) {
+                                 double* divout, int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
   // radial chunks of >= 32 planes, ~12k blocks: measured best at 6x128^2x256 and 6x256^2x384
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 12288, 32);
@@ -925,7 +957,7 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
   const int kind = (rhs_mode || stage == 0) ? 0 : (stage < 4 ? 1 : 2);
 #define SOLAR_STAGE_LAUNCH(K, M)                                                                                \
   hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, \
-                     dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, nullptr, nullptr, 0, 0)
+                     dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, ecols, nullptr, nullptr, 0, 0)
   if (tl_dc2) {
     if constexpr (sizeof(S) == 8) {
       if (kind == 0)
@@ -957,7 +989,7 @@ template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                                  hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
-                                 int div32, const double* tl_dc2, const double* tl_div) {
+                                 int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
   int ti = ls.stage_ti, tj = ls.stage_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 32x8 (64x8 within 2%), then 16x16
     ti = pick_ti(T.n);
@@ -966,7 +998,7 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
 #define SOLAR_STAGE_CASE(A, B) \
   if (ti == A && tj == B)      \
     return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, \
-                                   div32, tl_dc2, tl_div);
+                                   div32, tl_dc2, tl_div, ecols);
   SOLAR_STAGE_CASE(64, 8)
   SOLAR_STAGE_CASE(64, 4)
   SOLAR_STAGE_CASE(32, 8)
@@ -977,23 +1009,25 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   ti = pick_ti(T.n);
   if (ti == 64)
     return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32, tl_dc2, tl_div);
+                                    div32, tl_dc2, tl_div, ecols);
   if (ti == 32)
     return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32, tl_dc2, tl_div);
+                                    div32, tl_dc2, tl_div, ecols);
   return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                   div32, tl_dc2, tl_div);
+                                   div32, tl_dc2, tl_div, ecols);
 }
 
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout, int div32,
-                        const double* tl_dc2, const double* tl_div) {
+                        const double* tl_dc2, const double* tl_div, void* ecols) {
   if (f32)
     return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
-                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div);
+                          gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div,
+                          static_cast<float*>(ecols));
   return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
-                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div);
+                        gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div,
+                        static_cast<double*>(ecols));
 }
 
 template <int TI, int TJ, typename S>
@@ -1068,15 +1102,15 @@ static hipError_t launch_stage_ens_t(const EnsMember* M, const int4* items, int 
   dim3 g(grid), block(TI, TJ);
   if (stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
                        nitems, cur);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
                        nitems, cur);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, M, items,
+                       nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
                        nitems, cur);
   return hipGetLastError();
 }

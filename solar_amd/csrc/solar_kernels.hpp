@@ -48,18 +48,22 @@ struct LaunchShape {
 };
 
 // State pointers are float* when f32 is set (FP32-state variant), double* otherwise.
+// ecols (optional): the edge_cols copies of q's W/E edge columns; the
+// column strips are then read from there.
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
-                         double* gys, double* gxv, double* gyv, hipStream_t st);
+                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols = nullptr);
 // c2f (optional): a 3-D c^2 field (n,n,nr,6) used instead of the radial
 // profile; divout (optional): receives div(v) of the stage input (n,n,nr,6),
 // as double, or as float when div32 is set.  tl_dc2, tl_div (optional, both
 // or neither): tangent-linear stage, adding -dc2*rho0*div_primal to the
-// pressure equation (dc2 and div_primal fields (n,n,nr,6)).
+// pressure equation (dc2 and div_primal fields (n,n,nr,6)).  ecols
+// (optional, not with rhs_mode): receives the output's edge columns in the
+// edge_cols layout (p', v_xi, v_eta; all five fields after the last stage).
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls, const double* c2f = nullptr,
                         double* divout = nullptr, int div32 = 0, const double* tl_dc2 = nullptr,
-                        const double* tl_div = nullptr);
+                        const double* tl_div = nullptr, void* ecols = nullptr);
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
                          const double* fgy, hipStream_t st, const LaunchShape& ls);
 // Ensemble launches.  items: 4 ints per work item, (member, bi | bj << 16,
@@ -83,6 +87,12 @@ constexpr int gather_blocks_per_edge(int nr) { return (nr + gather_kc(nr) - 1) /
 // two each (n <= 512).
 constexpr int gather_threads_max(int ni) { return ni == 1 ? 1024 : 768; }
 inline int gather_threads(int n, int ni) { return (((3 * n + ni - 1) / ni) + 63) / 64 * 64; }
+// Compact copies of the edge columns a gather reads from the W/E edges
+// (columns i = 1..3 and n-4..n-2 of every row): edge_cols[P][side W/E][k]
+// [field 0..4][j][layer 1..3], so one strip of one field and plane is 3n
+// contiguous values.  Elements per (panel, side, plane), and in all.
+__host__ __device__ constexpr size_t edge_cols_plane(int n) { return (size_t)15 * n; }
+inline size_t edge_cols_elems(int n, int nr) { return (size_t)12 * nr * edge_cols_plane(n); }
 // mode 0: count of non-finite values -> acc[0]; mode 1: sum of squares of
 // `field` over all panels -> acc[1].
 hipError_t launch_scan(const void* q, bool f32, size_t nf, double* acc, int mode, hipStream_t st, int field = 3);

@@ -170,6 +170,22 @@ def test_graph_and_eager_launches_agree(t0, monkeypatch):
         assert np.array_equal(s.get_state(), a)
 
 
+@pytest.mark.parametrize("bits", [64, 32])
+def test_gathers_from_state_and_from_edge_columns_agree(t0, monkeypatch, bits):
+    """The W/E strips read from the stages' edge-column copies (the default)
+    and read from the state itself (SOLAR_NO_EDGE_COLS) give the same steps."""
+    cfg, _, _, q0, dt = t0
+    bg = make_inputs(cfg)[0]
+    out = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("SOLAR_NO_EDGE_COLS", flag)
+        with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits) as x:
+            x.set_state(q0)
+            x.rk4(3, dt)
+            out.append(x.get_state())
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("n,nr,kind,steps", [(48, 20, 1, 2), (64, 32, 1, 2), (128, 24, 0, 1)])
 def test_rk4_tile_shapes(n, nr, kind, steps):
     """Every tile shape the launcher picks (16x16, 32x8, 64x4) and chunked radial marching."""
@@ -192,6 +208,32 @@ def test_rk4_tile_shapes(n, nr, kind, steps):
         gx, gy = s.vector_ghosts(q[:, 1], q[:, 2])
         ox, oy = o.vector_ghosts(q[:, 1], q[:, 2])
         assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+
+
+@pytest.mark.parametrize("n,nr,env", [
+    (32, 131, {}),  # stage chunks of 33, 33, 33, 32 planes; filter chunks of 66, 65
+    (16, 13, {"SOLAR_STAGE_CHUNKS": "12", "SOLAR_FILTER_CHUNKS": "12"}),  # 2-plane chunks, the last five empty
+])
+def test_rk4_uneven_radial_chunks(n, nr, env, monkeypatch):
+    """Radial chunks of unequal length (nr not a multiple of the chunk count),
+    chunks shorter than the stencils' radial reach, and chunk counts that
+    leave trailing chunks with no planes."""
+    from solar_amd import background, generate_ic
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    bg = background(1, nr)
+    L = max(1, round(0.08 * nr))
+    q0, _ = generate_ic(1, n, nr, bg[0], bg[1], 5, [1e-3, 1e-6, 8])
+    q0 = q0 + 1e-4 * np.random.default_rng(nr).standard_normal(q0.shape)
+    dt = 0.5 * min(bg[0][1] - bg[0][0], bg[0][0] * (np.pi / 2) / (n - 1))
+    print(f"[case] chunks n={n} nr={nr} env={env}", flush=True)
+    with Solver(n, nr, *bg, L, 0.1) as s:
+        s.set_state(q0)
+        s.rk4(2, dt)
+        got = s.get_state()
+        o = oracle.Oracle(n, nr, *bg, L, 0.1)
+        want, _ = o.rk4(q0, 2, dt)
+        assert _assert_parity(got, want, f"rk4 n={n} nr={nr} {env}")
 
 
 @pytest.mark.parametrize("n,nr", [(352, 13), (512, 12), (64, 45)])
