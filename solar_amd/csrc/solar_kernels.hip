@@ -225,12 +225,19 @@ struct StagePlane {
 // compiler.  2 keeps every variant within 256 registers: left alone, the
 // ensemble and tangent-linear middle stages take 254-255 VGPRs plus 2 AGPRs
 // and drop to one wave per SIMD (measured: ensemble step 44.3 -> 42.4 ms,
-// the single-grid stages unchanged).
+// the single-grid stages unchanged).  The single-grid FP32-state variants
+// get 3 (168 registers): the middle stage then spills 20 bytes per lane, and
+// still runs faster with the third wave (measured at 6x128^2x256: 0.537 ->
+// 0.501 ms per stage, step 7.57e9 -> 7.92e9 grid-point steps/s).
 #ifndef SOLAR_STAGE_WAVES
 #define SOLAR_STAGE_WAVES 2
 #endif
+#ifndef SOLAR_STAGE_WAVES_F32
+#define SOLAR_STAGE_WAVES_F32 3
+#endif
 #if SOLAR_STAGE_WAVES > 0
-#define SOLAR_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(SOLAR_STAGE_WAVES)))
+#define SOLAR_STAGE_ATTR \
+  __attribute__((amdgpu_waves_per_eu(sizeof(S) == 4 && MODE == 0 ? SOLAR_STAGE_WAVES_F32 : SOLAR_STAGE_WAVES)))
 #else
 #define SOLAR_STAGE_ATTR
 #endif
