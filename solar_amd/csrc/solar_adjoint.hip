@@ -433,11 +433,20 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
 // holds its own in registers.  RADF applies FILTER_R^T (filter.f90:682-693)
 // to the per-plane ghost adjoints from an 11-plane register window, one new
 // plane per step.
-// Threads per block: vector strips (whose per-point tables take ~70
-// registers) at most 512, scalar strips as the forward gather.
-constexpr int gathert_threads_max(bool vec, int ni) { return vec ? 512 : gather_threads_max(ni); }
+// Threads per block: one strip point per thread up to 1024 (n <= 341);
+// beyond, vector strips (whose per-point tables take ~70 registers) use
+// three points per thread at 512 threads, scalar strips two at 768.
+// The next plane's ghost values load while this plane is summed, in two
+// alternating buffers (the loop is unrolled by two), so no register copy
+// between planes waits on the loads in flight.  Each plane's read-modify-
+// write inputs are issued first in its step: the stores wait for them
+// without draining the ghost prefetch issued after them.  A thread
+// past the strip (val false) takes strip point 0's indices and tables: its
+// loads, LDS writes and stores repeat that point's, with the same values, so
+// none of them needs a branch.  HASW: Wn is not null.
+constexpr int gathert_threads_max(bool vec, int ni) { return ni == 1 ? 1024 : (vec ? 512 : gather_threads_max(ni)); }
 
-template <bool VEC, bool RADF, int NI>
+template <bool VEC, bool RADF, int NI, bool HASW>
 __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
     DevTables T, const double* __restrict__ gxs, const double* __restrict__ gys, const double* __restrict__ gxv,
     const double* __restrict__ gyv, double* __restrict__ Y, double* __restrict__ Wn, double beta, int sfield,
@@ -457,29 +466,51 @@ __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const size_t fo = (size_t)fld * field_stride_in;
   const size_t kst = (size_t)3 * n;  // plane stride of both ghost layouts
-  const double* gsrc = (e < 2) ? gxs : gys;
-  const double* vsrc = (e < 2) ? gxv : gyv;
+  const auto gsrc = to_global((e < 2) ? gxs : gys);
+  const auto vsrc = to_global((e < 2) ? (VEC ? gxv : gxs) : (VEC ? gyv : gys));
   const size_t gside = fo + (size_t)(P * 2 + (e < 2 ? e : e - 2)) * nr * kst;
   const bool col = (se == kW || se == kE);  // column strips: layer index fastest in memory
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
 
   // this thread's receiver ghosts (staging) and strip points (sums)
-  bool val[NI];
-  int slot[NI], gmp[NI][kGtWidth], idx[NI];
   size_t gofs[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int t = threadIdx.x + it * blockDim.x;
+    gofs[it] = gside + (t < strip ? t : 0);
+  }
+  // Ghost inputs of one plane k (RADF: of plane k+5, the one entering the
+  // window; zeroed where it is used when k+5 lies beyond the filtered planes).
+  struct In {
+    double g[NI], gv[NI];
+  };
+  auto load_in = [&](int k, In& d) {
+    const int kk = min(k, k1 - 1);  // past the chunk: loaded and not used
+    const int kg = RADF ? min(kk + 5, nr - 1 - kNstr) : kk;
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      d.g[it] = gsrc[gofs[it] + (size_t)kg * kst];
+      if (VEC) d.gv[it] = vsrc[gofs[it] + (size_t)kk * kst];
+    }
+  };
+  // The first plane's ghosts load before the tables, so that the loop is
+  // entered with as many operations issued after them as on its back edge
+  // (the plane's stores): the wait for them at the top of the loop then
+  // leaves the stores in flight.
+  In cur, nxt;
+  load_in(k0, nxt);
+  int slot[NI], gmp[NI][kGtWidth], idx[NI];
   double ab[NI][2], gw[NI][kGtWidth], c[NI][6];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
     const int t = threadIdx.x + it * blockDim.x;
-    val[it] = t < strip;
-    const int tt = val[it] ? t : 0;
+    const int tt = t < strip ? t : 0;
     // x ghosts [m][l] (l fastest), y ghosts [l][m]
     const int l = (e < 2) ? tt % 3 + 1 : tt / n + 1;
     const int m = (e < 2) ? tt / 3 : tt % n;
     const int mp = rev ? n - 1 - m : m;
     slot[it] = (l - 1) * n + mp;
-    gofs[it] = gside + tt;
     if (VEC) {
       const double* abp = T.trans + ((((size_t)s * 4 + se) * 3 + (l - 1)) * n + mp) * 2;
       ab[it][0] = abp[0];
@@ -499,52 +530,51 @@ __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
       for (int x = 0; x < 6; ++x) c[it][x] = cv[(size_t)idx[it] * 6 + x];
     }
   }
-  double* ys = Y + (size_t)(s * 5 + sfield + fld) * nf;
-  double* ws = Wn ? Wn + (size_t)(s * 5 + sfield + fld) * nf : nullptr;
-  double* y1 = Y + (size_t)(s * 5 + 1) * nf;
-  double* y2 = y1 + nf;
-  double* w1 = Wn ? Wn + (size_t)(s * 5 + 1) * nf : nullptr;
-  double* w2 = Wn ? w1 + nf : nullptr;
+  const auto ys = to_global(Y + (size_t)(s * 5 + sfield + fld) * nf);
+  const auto y1 = to_global(Y + (size_t)(s * 5 + 1) * nf);
+  const auto y2 = y1 + nf;
+  // without Wn these alias Y and are never loaded or stored
+  const auto ws = to_global(HASW ? Wn + (size_t)(s * 5 + sfield + fld) * nf : Y);
+  const auto w1 = to_global(HASW ? Wn + (size_t)(s * 5 + 1) * nf : Y);
+  const auto w2 = w1 + nf;
 
   // RADF: window of per-plane ghost adjoints over planes k-5..k+5 (zero
   // outside the filtered planes [5, nr-6], which FILTER_R leaves alone)
   double win[NI][11];
-  auto gat = [&](int it, int kk) {
-    return (kk >= kNstr && kk <= nr - 1 - kNstr) ? gsrc[gofs[it] + (size_t)kk * kst] : 0.0;
-  };
   if (RADF) {
 #pragma unroll
     for (int it = 0; it < NI; ++it)
 #pragma unroll
-      for (int q = 0; q < 10; ++q) win[it][q + 1] = gat(it, k0 - 5 + q);
+      for (int q = 0; q < 10; ++q) {
+        const int kk = k0 - 5 + q;
+        win[it][q + 1] = (kk >= kNstr && kk <= nr - 1 - kNstr) ? gsrc[gofs[it] + (size_t)kk * kst] : 0.0;
+      }
   }
-  for (int k = k0; k < k1; ++k) {
-    double* buf = sg + ((k - k0) & 1) * bstride;
+  auto step = [&](int k, In& cur, In& nxt) {
     const size_t kp = (size_t)k * plane;
-    // read-modify-write inputs of this plane, issued before the barrier
     double yv[NI], wv[NI], y1v[NI], y2v[NI], w1v[NI], w2v[NI];
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
-      if (!val[it]) continue;
-      yv[it] = ys[kp + idx[it]];
-      if (ws) wv[it] = ws[kp + idx[it]];
+      const size_t x = kp + idx[it];
+      yv[it] = ys[x];
+      if (HASW) wv[it] = ws[x];
       if (VEC) {
-        y1v[it] = y1[kp + idx[it]];
-        y2v[it] = y2[kp + idx[it]];
-        if (w1) {
-          w1v[it] = w1[kp + idx[it]];
-          w2v[it] = w2[kp + idx[it]];
+        y1v[it] = y1[x];
+        y2v[it] = y2[x];
+        if (HASW) {
+          w1v[it] = w1[x];
+          w2v[it] = w2[x];
         }
       }
     }
+    double* buf = sg + ((k - k0) & 1) * bstride;
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
-      if (!val[it]) continue;
       double gsv;
       if (RADF) {
 #pragma unroll
         for (int q = 0; q < 10; ++q) win[it][q] = win[it][q + 1];
-        win[it][10] = gat(it, k + 5);
+        win[it][10] = (k + 5 <= nr - 1 - kNstr) ? cur.g[it] : 0.0;
         gsv = 0.0;  // sum over kk = k-5..k+5 ascending: filtr[k - kk + 5] = filtr[10 - q]
 #pragma unroll
         for (int q = 0; q < 11; ++q) {
@@ -552,19 +582,18 @@ __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
           if (kk >= kNstr && kk <= nr - 1 - kNstr) gsv += T.filtr[10 - q] * win[it][q];
         }
       } else {
-        gsv = gsrc[gofs[it] + (size_t)k * kst];
+        gsv = cur.g[it];
       }
       buf[slot[it]] = gsv;
       if (VEC) {
-        const double gv = vsrc[gofs[it] + (size_t)k * kst];
-        buf[strip + slot[it]] = ab[it][0] * gv;
-        buf[2 * strip + slot[it]] = ab[it][1] * gv;
+        buf[strip + slot[it]] = ab[it][0] * cur.gv[it];
+        buf[2 * strip + slot[it]] = ab[it][1] * cur.gv[it];
       }
     }
+    load_in(k + 1, nxt);
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
-      if (!val[it]) continue;
       double as = 0.0, ath = 0.0, aph = 0.0;
 #pragma unroll
       for (int q = 0; q < kGtWidth; ++q) {
@@ -577,18 +606,22 @@ __global__ __launch_bounds__(gathert_threads_max(VEC, NI)) void k_adj_gathert(
       }
       const size_t x = kp + idx[it];
       ys[x] = yv[it] + as;
-      if (ws) ws[x] = wv[it] + beta * as;
+      if (HASW) ws[x] = wv[it] + beta * as;
       if (VEC) {
         const double ax = c[it][0] * c[it][1] * ath + c[it][3] * c[it][4] * aph;
         const double ay = c[it][0] * c[it][2] * ath + c[it][3] * c[it][5] * aph;
         y1[x] = y1v[it] + ax;
         y2[x] = y2v[it] + ay;
-        if (w1) {
+        if (HASW) {
           w1[x] = w1v[it] + beta * ax;
           w2[x] = w2v[it] + beta * ay;
         }
       }
     }
+  };
+  for (int k = k0; k < k1; ++k) {
+    cur = nxt;
+    step(k, cur, nxt);
   }
 }
 
@@ -830,35 +863,42 @@ hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream
   return d32 ? launch_adj_stage_t<16, 16, true>(T, a, st) : launch_adj_stage_t<16, 16, false>(T, a, st);
 }
 
-template <bool VEC, bool RADF, int NI>
-static hipError_t launch_gathert(const DevTables& T, const double* gxs, const double* gys, const double* gxv,
-                                 const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
-                                 hipStream_t st) {
+template <bool VEC, bool RADF, int NI, bool HASW>
+static hipError_t launch_gathert_w(const DevTables& T, const double* gxs, const double* gys, const double* gxv,
+                                   const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
+                                   hipStream_t st) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
   dim3 grid((unsigned)(12 * gather_blocks_per_edge(T.nr)), (unsigned)nfields);
-  const int nt = VEC ? std::min(512, ((3 * T.n + NI - 1) / NI + 63) / 64 * 64) : gather_threads(T.n, NI);
+  const int nt = std::min(gathert_threads_max(VEC, NI), ((3 * T.n + NI - 1) / NI + 63) / 64 * 64);
   const size_t lds = sizeof(double) * 2 * (VEC ? 9 : 3) * (size_t)T.n;  // n <= 512: at most 72 KiB
   if (lds > 64 * 1024) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_gathert<VEC, RADF, NI>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_gathert<VEC, RADF, NI, HASW>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
     if (e != hipSuccess) return e;
   }
   for (int pass = 0; pass < 2; ++pass) {
-    hipLaunchKernelGGL((k_adj_gathert<VEC, RADF, NI>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn, beta,
-                       sfield, field_stride, pass);
+    hipLaunchKernelGGL((k_adj_gathert<VEC, RADF, NI, HASW>), grid, dim3(nt), lds, st, T, gxs, gys, gxv, gyv, Y, Wn,
+                       beta, sfield, field_stride, pass);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }
 
+template <bool VEC, bool RADF, int NI>
+static hipError_t launch_gathert(const DevTables& T, const double* gxs, const double* gys, const double* gxv,
+                                 const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
+                                 hipStream_t st) {
+  if (Wn) return launch_gathert_w<VEC, RADF, NI, true>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+  return launch_gathert_w<VEC, RADF, NI, false>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+}
+
 hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, const double* gys, const double* gxv,
                               const double* gyv, double* Y, double* Wn, double beta, int sfield, int nfields,
                               hipStream_t st, bool radial_filter) {
-  if (vec) {  // strip points per thread: 1 (n <= 170), 2 (n <= 341), 3 (n <= 512) at 512 threads
-    const int ni = (3 * T.n + 511) / 512;
-    if (ni == 1) return launch_gathert<true, false, 1>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
-    if (ni == 2) return launch_gathert<true, false, 2>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
+  if (vec) {  // strip points per thread: 1 (n <= 341, up to 1024 threads), 3 (n <= 512, 512 threads)
+    if (3 * T.n <= 1024)
+      return launch_gathert<true, false, 1>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
     return launch_gathert<true, false, 3>(T, gxs, gys, gxv, gyv, Y, Wn, beta, sfield, nfields, st);
   }
   const bool two = 3 * T.n > 1024;  // n > 341: two strip points per thread
