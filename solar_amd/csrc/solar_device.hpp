@@ -26,10 +26,12 @@ __device__ __forceinline__ global_ptr<T> to_global(T* p) {
 
 // Block index remap so that consecutive logical blocks (neighbouring tiles,
 // whose halos overlap) run on the same XCD and share its L2.  Hardware
-// dispatches block b to XCD b % 8.
+// dispatches block b to XCD b % 8; XCD x receives nb / 8 blocks, one more
+// when x < nb % 8, and is given the contiguous logical range that starts at
+// x * (nb / 8) + min(x, nb % 8).  A bijection on [0, nb) for every nb.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
-  if (nb % 8 != 0) return b;
-  return (b % 8) * (nb / 8) + b / 8;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
 }
 
 }  // namespace solar
