@@ -12,15 +12,17 @@
 //                  Tam-Dong rows transposed) applied to the transposed
 //                  pointwise terms; edge tiles emit the adjoints of the ghost
 //                  values they would have read.
-//   k_adj_scatter  the transpose of the panel gather: ghost adjoints pushed
-//                  through the receiver-basis transform, the 9-point Lagrange
-//                  weights and the (xi,eta)->(theta,phi) conversion into the
-//                  sender panel's near-edge rows (atomic adds: strips of one
-//                  panel meet at its corners).
-//   k_adj_filter_r / k_adj_filter_xe   FILTER_R^T, then FILTER_ETA^T and
-//                  FILTER_XI^T with the ghost adjoints of both passes (the
-//                  eta-pass ghosts come from the unfiltered field,
-//                  filter.f90:25-28,43).
+//   k_adj_gathert  the transpose of the panel gather, as a gather over the
+//                  sender's strips: ghost adjoints through the receiver-basis
+//                  transform, the transposed 9-point Lagrange rows and the
+//                  (xi,eta)->(theta,phi) conversion into the sender panel's
+//                  near-edge rows and columns.  S/N strips, then W/E strips:
+//                  strips of one pass share no point, so no add is atomic
+//                  and the results do not depend on scheduling.
+//   k_adj_filter   FILTER_R^T of the per-plane FILTER_XI^T FILTER_ETA^T.
+//   k_adj_filter_ghosts  the per-plane ghost adjoints of both angular
+//                  filter passes (the eta-pass ghosts come from the
+//                  unfiltered field, filter.f90:25-28,43).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
