@@ -72,9 +72,10 @@ def test_filter_transpose(n, nr, kind):
 
 @pytest.mark.parametrize("n,nr", [(176, 13), (352, 12), (512, 12)])
 def test_transposes_wide_panels(n, nr):
-    """L and F transposes where the transposed gathers hold 2 (n = 176) and 3
-    (n = 352, 512) strip points per thread; at n = 512 the staged lines take
-    more than 64 KiB of LDS."""
+    """L and F transposes on wide panels: the transposed gathers run one strip
+    point per thread up to 1024 threads (n = 176), and beyond n = 341 three
+    (vector strips) or two (scalar strips) per thread (n = 352, 512); at
+    n = 512 the staged lines take more than 64 KiB of LDS."""
     cfg, bg, _ = _setup(n, nr, 1)
     rng = np.random.default_rng(n + nr)
     x = rng.standard_normal((6, 5, nr, n, n))
