@@ -115,6 +115,10 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
   // alternating by plane, so one barrier per plane suffices.
   __shared__ double sxv[2][TJ][TI + 6], sxp[2][TJ][TI + 6];
   __shared__ double syv[2][TJ + 6][TI], syp[2][TJ + 6][TI];
+  // DRP taps for the edge tiles' ghost adjoints, which index them per
+  // thread: from LDS, a dynamic index into the kernel argument would be a
+  // vector load that the plane loop waits on together with the prefetch
+  __shared__ double sfc[2][7];
   constexpr int NT = TI * TJ, NHX = 6 * TJ, NH = 6 * TJ + 6 * TI;
   static_assert(NH <= NT, "one halo cell per thread at most");
 
@@ -244,6 +248,10 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     L.c2h = c2b[c2f ? hk : 0];
     L.c2r = c2b[c2f ? ek : 0];
   };
+  if (threadIdx.y == 0 && threadIdx.x < 7) {  // read after the first plane's barrier
+    sfc[0][threadIdx.x] = T.fcx[threadIdx.x];
+    sfc[1][threadIdx.x] = T.fcy[threadIdx.x];
+  }
   AdjPlane nxt;
   load_plane(k0, nxt);
 
@@ -377,8 +385,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = tx + 1;
       double gp = 0.0, gv = 0.0;
       for (int ii = 0; ii <= 3 - l; ++ii) {
-        gp += T.fcx[3 - l - ii] * sxp[bf][ty][3 + ii];
-        gv += T.fcx[3 - l - ii] * sxv[bf][ty][3 + ii];
+        gp += sfc[0][3 - l - ii] * sxp[bf][ty][3 + ii];
+        gv += sfc[0][3 - l - ii] * sxv[bf][ty][3 + ii];
       }
       const size_t o = (size_t)(P * 2) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
       gxa_p[o] = sig_dt * gp;
@@ -388,8 +396,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = tx - (TI - 3) + 1;
       double gp = 0.0, gv = 0.0;
       for (int ii = n - 4 + l; ii <= n - 1; ++ii) {
-        gp += T.fcx[n + 2 + l - ii] * sxp[bf][ty][3 + ii - i0];
-        gv += T.fcx[n + 2 + l - ii] * sxv[bf][ty][3 + ii - i0];
+        gp += sfc[0][n + 2 + l - ii] * sxp[bf][ty][3 + ii - i0];
+        gv += sfc[0][n + 2 + l - ii] * sxv[bf][ty][3 + ii - i0];
       }
       const size_t o = (size_t)(P * 2 + 1) * gx_side + ((size_t)k * n + j) * 3 + (l - 1);
       gxa_p[o] = sig_dt * gp;
@@ -399,8 +407,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = ty + 1;
       double gp = 0.0, gv = 0.0;
       for (int jj = 0; jj <= 3 - l; ++jj) {
-        gp += T.fcy[3 - l - jj] * syp[bf][3 + jj][tx];
-        gv += T.fcy[3 - l - jj] * syv[bf][3 + jj][tx];
+        gp += sfc[1][3 - l - jj] * syp[bf][3 + jj][tx];
+        gv += sfc[1][3 - l - jj] * syv[bf][3 + jj][tx];
       }
       const size_t o = (size_t)(P * 2) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
       gya_p[o] = sig_dt * gp;
@@ -410,8 +418,8 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
       const int l = ty - (TJ - 3) + 1;
       double gp = 0.0, gv = 0.0;
       for (int jj = n - 4 + l; jj <= n - 1; ++jj) {
-        gp += T.fcy[n + 2 + l - jj] * syp[bf][3 + jj - j0][tx];
-        gv += T.fcy[n + 2 + l - jj] * syv[bf][3 + jj - j0][tx];
+        gp += sfc[1][n + 2 + l - jj] * syp[bf][3 + jj - j0][tx];
+        gv += sfc[1][n + 2 + l - jj] * syv[bf][3 + jj - j0][tx];
       }
       const size_t o = (size_t)(P * 2 + 1) * gy_side + ((size_t)k * 3 + (l - 1)) * n + i;
       gya_p[o] = sig_dt * gp;
