@@ -662,6 +662,9 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_adj_filter_ghosts(De
                                                                                double* __restrict__ fgxa,
                                                                                double* __restrict__ fgya) {
   extern __shared__ double sl[];  // 2 buffers x 3 lines x n
+  // filter taps, indexed per thread: from LDS (a dynamic index into the
+  // kernel argument is a vector load, waited on with the prefetched plane)
+  __shared__ double sft[7];
   const int n = T.n, nr = T.nr, strip = 3 * n;
   const int kc = gather_kc(nr);
   const int b = blockIdx.x;
@@ -674,15 +677,16 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_adj_filter_ghosts(De
   const size_t fst = (size_t)12 * gx_side;
   const bool colw = e < 2;                       // W/E: three edge columns, all rows
   const int c0 = (e == 1) ? n - 3 : 0, r0 = (e == 3) ? n - 3 : 0;
+  // A thread past the strip takes point 0's indices: its loads, LDS writes
+  // and stores repeat that point's with the same values, so none of them
+  // sits under a branch.
   int src[NI], slot[NI];
-  bool val[NI];
   size_t ob[NI];
   int ol[NI], om[NI];
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
     const int t = threadIdx.x + it * blockDim.x;
-    val[it] = t < strip;
-    const int tt = val[it] ? t : 0;
+    const int tt = t < strip ? t : 0;
     if (colw) {  // row tt/3, column c0 + tt%3 -> line tt%3, along index tt/3
       src[it] = (tt / 3) * n + c0 + tt % 3;
       slot[it] = (tt % 3) * n + tt / 3;
@@ -703,33 +707,34 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_adj_filter_ghosts(De
     for (int it = 0; it < NI; ++it) nxt[it] = g[(size_t)k * plane + src[it]];
   };
   load(k0);
+  // every thread writes a tap, without a branch (read after the first
+  // plane's barrier)
+  sft[threadIdx.x % 7] = T.filt[threadIdx.x % 7];
   for (int k = k0; k < k1; ++k) {
     double* ln = sl + ((k - k0) & 1) * strip;
 #pragma unroll
-    for (int it = 0; it < NI; ++it)
-      if (val[it]) ln[slot[it]] = nxt[it];
+    for (int it = 0; it < NI; ++it) ln[slot[it]] = nxt[it];
     load(min(k + 1, k1 - 1));  // past the last plane: loaded and not used
     __syncthreads();
     const bool live = k >= kNstr && k <= nr - 1 - kNstr;
 #pragma unroll
     for (int it = 0; it < NI; ++it) {
-      if (!val[it]) continue;
       const int l = ol[it], m = om[it];
       double s = 0.0;
       if (live) {
         if (e == 2) {  // rows 0..2 at column m
-          for (int jj = 0; jj <= 3 - l; ++jj) s += T.filt[3 - l - jj] * ln[jj * n + m];
+          for (int jj = 0; jj <= 3 - l; ++jj) s += sft[3 - l - jj] * ln[jj * n + m];
         } else if (e == 3) {  // rows n-3..n-1 at column m
-          for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += T.filt[n + 2 + l - jj] * ln[(jj - r0) * n + m];
+          for (int jj = n - 4 + l; jj <= n - 1; ++jj) s += sft[n + 2 + l - jj] * ln[(jj - r0) * n + m];
         } else {  // abar on the edge columns at row m
           const int ia = (e == 0) ? 0 : n - 4 + l, ib = (e == 0) ? 3 - l : n - 1;
           for (int ii = ia; ii <= ib; ++ii) {
             double av = 0.0;  // abar(ii, m) = sum_q f[6-q] bbar(ii, m-3+q), zero beyond the panel
             for (int q = 0; q < 7; ++q) {
               const int jj = m - 3 + q;
-              if (jj >= 0 && jj < n) av += T.filt[6 - q] * ln[(ii - c0) * n + jj];
+              if (jj >= 0 && jj < n) av += sft[6 - q] * ln[(ii - c0) * n + jj];
             }
-            s += ((e == 0) ? T.filt[3 - l - ii] : T.filt[n + 2 + l - ii]) * av;
+            s += ((e == 0) ? sft[3 - l - ii] : sft[n + 2 + l - ii]) * av;
           }
         }
       }
