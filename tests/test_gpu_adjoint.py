@@ -17,8 +17,9 @@ from solar_amd import RHS_BC, RHS_SPONGE, Config, Solver, background, generate_i
 
 pytestmark = pytest.mark.gpu
 
-# (n, nr, background kind): n = 32 takes the 32x8 adjoint tiles, 16 and 48 the 16x16 ones
-CASES = [(16, 24, 1), (32, 40, 1), (48, 32, 0)]
+# (n, nr, background kind): n = 32 takes the 32x8 adjoint tiles, 16 and 48 the 16x16 ones;
+# nr = 100 splits the radial march (4 stage chunks of 25 planes, 2 filter chunks of 50)
+CASES = [(16, 24, 1), (32, 40, 1), (48, 32, 0), (16, 100, 1)]
 
 
 def _setup(n, nr, kind, steps=1):
