@@ -246,8 +246,8 @@ def main():
                          "of each field of the FP32 run against the FP64 run")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, nothing runs it)", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the panels across GPUs is a design only; no code runs it)", file=sys.stderr)
         sys.exit(2)
 
     if args.only == "adjoint":
