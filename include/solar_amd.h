@@ -184,7 +184,8 @@ int solar_tangent_linear(solar_ctx* ctx, const double* q0, int nsteps, double dt
  * the modelled traces; *segment_used (may be NULL) the segment length.
  * Overwrites the handle's state.  Device memory beyond the handle's:
  * segment * 40 B per point for the divergences plus one state per earlier
- * segment. */
+ * segment.  No sum depends on scheduling (no atomics): the same inputs give
+ * the same bits, whatever checkpoint_every. */
 int solar_misfit_gradient(solar_ctx* ctx, const double* q0, int nsteps, double dt, const double* data,
                           int checkpoint_every, double* J, double* grad, double* seis, int* segment_used);
 
