@@ -11,9 +11,10 @@ from solar_amd import Config, Ensemble, Solver, background, generate_ic
 
 pytestmark = pytest.mark.gpu
 
-# (n, nr, background kind, pulse width, seed)
+# (n, nr, background kind, pulse width, seed); nr = 67 and 13 are odd (67 splits into radial
+# chunks), and n = 96 sets the widest gather blocks every member's gather runs with
 MEMBERS = [(16, 24, 0, 0.05, 1), (32, 40, 1, 0.03, 2), (48, 24, 1, 0.08, 3), (16, 36, 0, 0.02, 4),
-           (32, 24, 0, 0.06, 5), (64, 20, 1, 0.04, 6)]
+           (32, 24, 0, 0.06, 5), (64, 20, 1, 0.04, 6), (32, 67, 1, 0.05, 7), (96, 13, 0, 0.04, 8)]
 STEPS = 6
 
 
