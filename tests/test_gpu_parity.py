@@ -236,6 +236,25 @@ def test_rk4_uneven_radial_chunks(n, nr, env, monkeypatch):
         assert _assert_parity(got, want, f"rk4 n={n} nr={nr} {env}")
 
 
+@pytest.mark.parametrize("n", [352, 512])
+def test_rk4_widest_panels(n):
+    """A full step where the gathers hold two strip points per thread
+    (3n > 1024) and read the stages' edge-column copies; at n = 512 their
+    staged lines take more than 64 KiB of LDS."""
+    from solar_amd import background, generate_ic
+    nr = 12
+    bg = background(1, nr)
+    q0, _ = generate_ic(1, n, nr, bg[0], bg[1], 3, [1e-3, 1e-6, 8])
+    dt = 0.5 * min(bg[0][1] - bg[0][0], bg[0][0] * (np.pi / 2) / (n - 1))
+    print(f"[case] widest n={n} nr={nr}", flush=True)
+    want, _ = oracle.Oracle(n, nr, *bg, 1, 0.1).rk4(q0, 1, dt)
+    with Solver(n, nr, *bg, 1, 0.1) as s:
+        s.set_state(q0)
+        s.rk4(1, dt)
+        got = s.get_state()
+    assert _assert_parity(got, want, f"rk4 n={n} nr={nr}")
+
+
 @pytest.mark.parametrize("n,nr", [(352, 13), (512, 12), (64, 45)])
 def test_ghosts_plane_chunks(n, nr):
     """Ghost gathers with two strip points per thread (3n > 1024; at n = 512
