@@ -60,11 +60,13 @@ def _float_storage_steps(o, q0, steps, dt):
     return q
 
 
-@pytest.mark.parametrize("name,n,nr,steps", [("t0", None, None, 3), ("tiles", 48, 20, 2), ("chunks", 32, 131, 1)])
+@pytest.mark.parametrize("name,n,nr,steps", [("t0", None, None, 3), ("tiles", 48, 20, 2), ("chunks", 32, 131, 1),
+                                              ("wide", 352, 12, 1)])
 def test_fp32_steps_equal_float_rounded_restatement(name, n, nr, steps):
     """The FP32-state steps are bit-identical to the CPU restatement with the
     same float rounding of the stored state (at the smoke grid, a grid of
-    16x16 tiles, and one with uneven radial chunks)."""
+    16x16 tiles, one with uneven radial chunks, and one whose gathers hold two
+    strip points per thread)."""
     from solar_amd import background, generate_ic
     if name == "t0":
         cfg = CONFIGS["t0"]
