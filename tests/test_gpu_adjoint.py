@@ -73,7 +73,7 @@ def test_filter_transpose(n, nr, kind):
 
 @pytest.mark.parametrize("n,nr", [(176, 13), (352, 12), (512, 12)])
 def test_transposes_wide_panels(n, nr):
-    """L and F transposes on wide panels: the transposed gathers run one strip
+    """L, F and S transposes on wide panels: the transposed gathers run one strip
     point per thread up to 1024 threads (n = 176), and beyond n = 341 three
     (vector strips) or two (scalar strips) per thread (n = 352, 512); at
     n = 512 the staged lines take more than 64 KiB of LDS."""
@@ -87,6 +87,13 @@ def test_transposes_wide_panels(n, nr):
         assert gap < 1e-13
         gap, a, b = _transpose_gap(s.filter_exp(x), w, x, s.adj_filter(w))
         print(f"[adj] F n={n} nr={nr}: gap={gap:.2e}", flush=True)
+        assert gap < 1e-13
+        # the whole step, whose transposed gathers also carry the W update
+        dt = _setup(n, nr, 1)[2]
+        s.set_state(x)
+        s.rk4(1, dt)
+        gap, a, b = _transpose_gap(s.get_state(), w, x, s.adj_step(w, dt))
+        print(f"[adj] S n={n} nr={nr}: gap={gap:.2e}", flush=True)
         assert gap < 1e-13
 
 
