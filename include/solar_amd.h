@@ -40,18 +40,18 @@ enum {
 
 /* Flags of solar_compute_rhs. */
 #define SOLAR_RHS_BC 1     /* apply BC_RHS (bc_acoustic_wave.f90:14-34) */
-#define SOLAR_RHS_SPONGE 2 /* apply SPONGE (sponge.f90:215-249) */
+#define SOLAR_RHS_SPONGE 2 /* apply SPONGE (sponge.f90:20-50) */
 
 /*
  * Set-up of the time-invariant tables and device buffers.  Replaces
  * SETUP_COORDINATES_METRICS_ALL (metric.f90:8), SETUP_LAGRANGE_INTERPOLANT
  * (interpolation.f90:9), the partner-rank tables of INITIALIZE_MPI
  * (init_mpi.f90:164-304) and the background arrays SOLAR_DATA fills
- * (ic_acoustic_wave.f90:257-271, radial-only there, so passed as profiles).
+ * (ic_acoustic_wave.f90:94-108, radial-only there, so passed as profiles).
  *   n          angular points per panel edge (nx = ny), multiple of 16
  *   nr         radial points, >= 12
  *   gr, c2, rho0, g0   radial profiles, length nr (nondimensional)
- *   sponge_len, a_sponge   sponge.f90:215,222 (k_sponge_length, a_sponge)
+ *   sponge_len, a_sponge   sponge.f90:27,20 (k_sponge_length, a_sponge)
  */
 int solar_init(solar_ctx** ctx, int n, int nr, const double* gr, const double* c2, const double* rho0,
                const double* g0, int sponge_len, double a_sponge);

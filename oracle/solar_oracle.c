@@ -15,7 +15,7 @@
  *   grad.f90:13-136                    COMPUTE_GRADIENT, COMPUTE_DIVERGENCE
  *   filter.f90:9-781                   FILTER_EXP, FILTER_XI/ETA/R
  *   compute_rhs_acoustic.f90:18-149    RHS (time-dependent forcing omitted)
- *   bc_acoustic_wave.f90:14-34, sponge.f90:215-249, timestep.f90:61-82
+ *   bc_acoustic_wave.f90:14-34, sponge.f90:20-50, timestep.f90:61-82
  * The MPI panel exchange (derivative.f90:55-133,374-452; filter.f90:93-171,
  * 365-443; partner ranks init_mpi.f90:164-304) becomes a copy of the sender
  * panel's strip argument into the receiver's ghost array, see route_x/route_y.
@@ -216,7 +216,7 @@ void orc_destroy(orc_t* o) {
 }
 
 /* The reference's c2 is the 3-D array c2(mx,my,mr) (compute_rhs_acoustic.f90:134,
- * ic_acoustic_wave.f90:257-271 fills it from a radial profile).  A non-NULL
+ * ic_acoustic_wave.f90:94-108 fills it from a radial profile).  A non-NULL
  * field c2f(N,N,R,6) replaces the radial profile in COMPUTE_RHS. */
 void orc_set_c2_field(orc_t* o, const double* c2f) {
   const size_t cnt = (size_t)6 * o->N * o->N * o->R;
@@ -834,7 +834,7 @@ void orc_bc_rhs(const orc_t* o, double* rhs) {
     }
 }
 
-/* sponge.f90:215-245: single-precision ratio, applied to fields 1, 4, 5. */
+/* sponge.f90:20-50: single-precision ratio, applied to fields 1, 4, 5. */
 double orc_sponge_sigma(const orc_t* o, int gk /* 1-based */) {
   const int L = o->sponge_len;
   const int lower = L, upper = o->R - L + 1;

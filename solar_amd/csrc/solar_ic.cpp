@@ -2,7 +2,7 @@
 //
 // The reference reads its background from a `solar_model` text file and
 // nondimensionalizes it by the values at the first radial point
-// (ic_acoustic_wave.f90:236-271), and fills q in GENERATE_IC
+// (ic_acoustic_wave.f90:60-108), and fills q in GENERATE_IC
 // (ic_acoustic_wave.f90:4-29, ic_gaussian_pulse.f90:1-30).  These are the
 // caller-side inputs of the hot path; this file produces them from closed
 // forms with a counter-based RNG, so the same seed gives the same bits on
@@ -96,7 +96,7 @@ extern "C" {
 //   kind 1: stratified, c^2 = (1.05-r)/(1.05-0.7), rho0 = (c^2)^1.5,
 //           hydrostatic g0 = 2.5/(0.35*Gamma1) with Gamma1 = 5/3 (the
 //           pressure p0 = rho0 c^2/Gamma1 then satisfies dp0/dr = -rho0 g0).
-// Units follow ic_acoustic_wave.f90:236-266: radius in units of the outer
+// Units follow ic_acoustic_wave.f90:73-108: radius in units of the outer
 // radius, speeds in units of c(r_min), densities in units of rho0(r_min).
 int solar_background(int kind, int nr, double* gr, double* c2, double* rho0, double* g0) {
   if (nr < 2 || (kind != 0 && kind != 1)) return -1;

@@ -564,7 +564,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     if (do_bc && (k == 0 || k == nr - 1)) {  // bc_acoustic_wave.f90:14-34
       r1 = 0.0; r2 = 0.0; r3 = 0.0; r4 = 0.0; r5 = 0.0;
     }
-    if (do_sp) {  // sponge.f90:243-245, after BC_RHS (timestep.f90:65-66)
+    if (do_sp) {  // sponge.f90:48-50, after BC_RHS (timestep.f90:65-66)
       const double sg = row.sigma;
       r1 = r1 - sg * (q1 - 0.0);
       r4 = r4 - sg * (rc - 0.0);

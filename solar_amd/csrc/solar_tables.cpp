@@ -26,7 +26,7 @@ const double kFiltCoeffsR[11] = {0.0009765, -0.0097655, 0.043945, -0.1171875, 0.
                                  0.2050775, -0.1171875, 0.043945, -0.0097655, 0.0009765};
 
 double sponge_sigma(int gk, int nr, int sponge_len, double a_sponge) {
-  // sponge.f90:222-241: REAL() ratios are single precision, the cube is taken
+  // sponge.f90:27-46: REAL() ratios are single precision, the cube is taken
   // in double precision with a real exponent.
   const int lower = sponge_len, upper = nr - sponge_len + 1;
   volatile float num, den;

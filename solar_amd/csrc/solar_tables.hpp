@@ -37,7 +37,7 @@ struct RadialRow {
   double g0;       //                              :83
   double rho0g0;   // rho0*g0                      :146
   double drho0dr;  // DDR(rho0)                    :51
-  double sigma;    // sponge coefficient           sponge.f90:235-241
+  double sigma;    // sponge coefficient           sponge.f90:40-46
   double w[7];     // DDR weights on planes base..base+6 (derivative.f90:656-754)
   double inv_rho0;  // 1/rho0(k) (used by the adjoint only)
 };
@@ -104,7 +104,7 @@ extern const double kFiltCoeffsR[11];
 bool build_tables(int n, int nr, const double* gr, const double* c2, const double* rho0, const double* g0,
                   int sponge_len, double a_sponge, Tables* t, std::string* err);
 
-// Sponge coefficient for 1-based radial index gk (sponge.f90:222-241).
+// Sponge coefficient for 1-based radial index gk (sponge.f90:27-46).
 double sponge_sigma(int gk, int nr, int sponge_len, double a_sponge);
 
 }  // namespace solar

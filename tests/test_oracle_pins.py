@@ -15,8 +15,24 @@ a closed form, an invariant or an independent geometric construction:
   filters                transfer function, constant preservation, the
                          truncated radial sum 0.999998, zeroed end planes
   gradient/divergence    closed-form fields
+  RHS assembly           closed forms on the stratified background: buoyancy
+                         from rho' alone, v = r e_r (both continuity terms and
+                         the pressure equation), p' = r (a . x) (-grad p/rho0)
+  FILTER_ETA / FILTER_R  transfer functions of phase-shifted waves with analytic
+                         ghosts; a radial ramp (window centred on its plane)
+  FILTER_EXP             impulse responses against the composition written on
+                         extended lines, ghosts from the unfiltered field (Q3),
+                         velocities filtered as scalars (Q2)
   RK4 loop               4th-order amplification polynomial (Taylor series)
   sponge/BC              single-precision sigma closed form, field selection
+
+Each of these plausible mistakes, planted one at a time in a scratch copy of
+solar_oracle.c, fails at least one test here: the buoyancy sign, the sign of
+rho0 g0 v_r, a dropped v_r d(rho0)/dr, c2 in place of c2 rho0, grad p times
+rho0, swapped grad p components, eta-pass ghosts from the xi-filtered field,
+the radial filter output one plane off its window, FILTER_ETA's ghost layers
+reversed or taken from the opposite strip, and basis-transformed velocity
+ghosts in the filter.
 """
 import numpy as np
 import pytest
@@ -309,3 +325,223 @@ def test_rhs_linear_and_zero_preserving(t0_inputs):
     assert np.linalg.norm(lhs - rhs) / np.linalg.norm(rhs) < 1e-13
     z, _ = o.rk4(np.zeros(q0.shape), 2, dt)
     assert np.all(z == 0)
+
+
+# ----------------------------------------------------------------------------
+# Pointwise RHS assembly (compute_rhs_acoustic.f90:38-149) on the stratified
+# background, where rho0, c2 and g0 are all different from 1 and from each
+# other, so a dropped term, a flipped sign or c2 in place of c2*rho0 moves
+# the result by O(1).
+
+
+def _strat_oracle(n, nr):
+    gr, c2, rho0, g0 = background(1, nr)
+    return oracle.Oracle(n, nr, gr, c2, rho0, g0, 3), gr, c2, rho0, g0
+
+
+def test_rhs_density_only_is_buoyancy():
+    """rho' alone: d(v_r)/dt = -rho' g0 / rho0 exactly, every other tendency 0
+    (compute_rhs_acoustic.f90:75-83; div v = 0 and grad p = 0 exactly)."""
+    n, nr = 16, 24
+    o, gr, c2, rho0, g0 = _strat_oracle(n, nr)
+    rng = np.random.default_rng(5)
+    rho = rng.standard_normal((6, nr, n, n))
+    q = np.zeros((6, 5, nr, n, n))
+    q[:, 0] = rho
+    r = o.compute_rhs(q)
+    want = -((rho * g0[None, :, None, None]) / rho0[None, :, None, None])
+    assert np.array_equal(r[:, 3], want)
+    for f in (0, 1, 2, 4):
+        assert not np.any(r[:, f]), f
+
+
+def test_rhs_radial_velocity_closed_form():
+    """v = r e_r (div v = 3):
+         d rho'/dt = -3 rho0 - r d(rho0)/dr          (:38-60)
+         d p'/dt   = -3 c2 rho0 + rho0 g0 r          (:128-146)
+    with d(rho0)/dr of rho0 = u^1.5, u = (1.05 - r)/0.35, in closed form."""
+    n, nr = 16, 64
+    o, gr, c2, rho0, g0 = _strat_oracle(n, nr)
+    q = np.zeros((6, 5, nr, n, n))
+    q[:, 3] = gr[None, :, None, None]
+    r = o.compute_rhs(q)
+    u = (1.05 - gr) / 0.35
+    drho0 = -1.5 * np.sqrt(u) / 0.35
+    want1 = -3 * rho0 - gr * drho0
+    want5 = -3 * c2 * rho0 + rho0 * g0 * gr
+    # DDR of rho0 is accurate to ~2e-5 at the outer closure; a dropped or
+    # sign-flipped term changes these by more than 0.1
+    assert np.abs(r[:, 0] - want1[None, :, None, None]).max() < 1e-4
+    assert np.abs(r[:, 4] - want5[None, :, None, None]).max() < 1e-10
+    for f in (1, 2, 3):
+        assert not np.any(r[:, f]), f
+
+
+def test_rhs_radial_velocity_with_3d_c2_field():
+    """The same with a 3-D c2(i,j,k) (the reference's c2 is a 3-D array, :134)."""
+    n, nr = 16, 32
+    o, gr, c2, rho0, g0 = _strat_oracle(n, nr)
+    rng = np.random.default_rng(9)
+    c2f = c2[None, :, None, None] * (1 + 0.3 * rng.random((6, nr, n, n)))
+    o.set_c2_field(c2f)
+    q = np.zeros((6, 5, nr, n, n))
+    q[:, 3] = gr[None, :, None, None]
+    r = o.compute_rhs(q)
+    want5 = -3 * c2f * rho0[None, :, None, None] + (rho0 * g0 * gr)[None, :, None, None]
+    assert np.abs(r[:, 4] - want5).max() < 1e-10
+
+
+def test_rhs_pressure_gradient_closed_form():
+    """p' = r (a . x) (grad p = a): dv/dt = -a / rho0 in the panel basis
+    (:62-73), d rho'/dt = d p'/dt = 0 exactly (v = 0)."""
+    n, nr = 32, 16
+    o, gr, c2, rho0, g0 = _strat_oracle(n, nr)
+    xyz = panel_points(n)
+    a = np.array([0.4, -0.9, 0.6])
+    q = np.zeros((6, 5, nr, n, n))
+    q[:, 4] = np.einsum("pjic,c->pji", xyz, a)[:, None] * gr[None, :, None, None]
+    r = o.compute_rhs(q)
+    g, _ = grid(n)
+    XI, ETA = np.meshgrid(g, g, indexing="xy")
+    rr = rho0[None, :, None, None]
+    for pp in range(6):
+        eth, eph, er = sph_basis(cube_points(pp, XI, ETA))
+        wx, we = theta_phi_to_xieta(pp, XI, ETA, eth @ a, eph @ a)
+        # the gradient itself is accurate to 2e-4 at n = 32
+        assert np.abs(r[pp, 1] * rr[0] + wx[None]).max() < 2e-4, pp
+        assert np.abs(r[pp, 2] * rr[0] + we[None]).max() < 2e-4, pp
+        assert np.abs(r[pp, 3] * rr[0] + (er @ a)[None]).max() < 1e-8, pp
+    assert not np.any(r[:, 0]) and not np.any(r[:, 4])
+
+
+# ----------------------------------------------------------------------------
+# Filters.  The 7-point filter is [1,-6,15,44,15,-6,1]/64 (header:99-105) and
+# the radial filter has the truncated decimals of header:123-133.
+
+FILT = np.array([1, -6, 15, 44, 15, -6, 1]) / 64.0
+FILT_R = np.array([0.0009765, -0.0097655, 0.043945, -0.1171875, 0.2050775, 0.753906,
+                   0.2050775, -0.1171875, 0.043945, -0.0097655, 0.0009765])
+
+
+def test_filter_eta_transfer_function():
+    """A wave along eta with its analytic ghosts comes out multiplied by
+    1 - sin^6(kappa/2), at every row including the three next to each edge
+    (filter.f90:448-629: interior, lower and upper boundary loops)."""
+    n, nr = 32, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    idx = np.arange(n)
+    lay = np.arange(1, 4)
+    for kappa, ph in [(0.3, 0.2), (1.1, 1.0), (2.0, -0.7), (np.pi, 0.4)]:
+        w = lambda x: np.cos(kappa * x + ph)
+        f = np.broadcast_to(w(idx)[None, :, None], (nr, n, n)).copy()
+        gp = np.broadcast_to(w(-lay)[None, :, None], (nr, 3, n)).copy()          # (N,3,R) [k][l][i]
+        gn = np.broadcast_to(w(n - 1 + lay)[None, :, None], (nr, 3, n)).copy()
+        out = o.filter_eta(f, gp, gn)
+        tf = 1 - np.sin(kappa / 2) ** 6
+        assert np.abs(out - tf * f).max() < 1e-14, kappa
+
+
+def test_filter_r_radial_wave_ramp_and_end_planes():
+    """Radial waves cos(kappa k + ph) come out multiplied by the truncated
+    filter's transfer function sum_m c_m cos(kappa m) on planes 6..R-5
+    (filter.f90:682-693); a ramp in k comes out as 0.999998 k, so the 11-tap
+    window is centred on its plane; planes 1-5 and R-4..R are 0 (:652)."""
+    n, nr = 10, 40
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    k = np.arange(1, nr + 1, dtype=float)
+    m = np.arange(-5, 6)
+    inner = slice(5, nr - 5)
+    for kappa, ph in [(0.4, 0.3), (1.3, -1.0), (2.6, 0.5)]:
+        g = np.broadcast_to(np.cos(kappa * k + ph)[:, None, None], (nr, n, n)).copy()
+        out = o.filter_r(g)
+        H = float(np.sum(FILT_R * np.cos(kappa * m)))
+        assert np.abs(out[inner] - H * g[inner]).max() < 1e-14, kappa
+        assert not np.any(out[:5]) and not np.any(out[nr - 5:])
+    ramp = np.broadcast_to(k[:, None, None], (nr, n, n)).copy()
+    out = o.filter_r(ramp)
+    assert np.abs(out[inner] - 0.999998 * ramp[inner]).max() < 1e-13
+
+
+def _filter_line(ext, coeffs, axis):
+    """Filter along `axis` of an array extended by its ghost layers: output
+    point i is the correlation of `coeffs` with ext[i : i + len(coeffs)]."""
+    win = np.lib.stride_tricks.sliding_window_view(ext, len(coeffs), axis=axis)
+    return win @ coeffs
+
+
+def _filter_exp_by_extension(f, gx, gy):
+    """FILTER_EXP as the reference composes it (filter.f90:23-46): ghosts of
+    the UNFILTERED field on all four edges; xi filter of the field on the line
+    extended by its W/E ghosts; eta filter of that result on the line extended
+    by the S/N ghosts; then the radial filter.  f (6,R,N,N); gx (6,2,R,N,3)
+    [W/E][k][j][layer]; gy (6,2,R,3,N) [S/N][k][layer][i]; layer 0 adjacent."""
+    nr = f.shape[1]
+    out = np.zeros_like(f)
+    for p in range(6):
+        ext = np.concatenate([gx[p, 0][..., ::-1], f[p], gx[p, 1]], axis=2)
+        t1 = _filter_line(ext, FILT, 2)
+        ext = np.concatenate([gy[p, 0][:, ::-1, :], t1, gy[p, 1]], axis=1)
+        t2 = _filter_line(ext, FILT, 1)
+        out[p, 5:nr - 5] = _filter_line(t2, FILT_R, 0)
+    return out
+
+
+def test_filter_exp_impulse_uses_unfiltered_ghosts():
+    """An impulse next to a panel edge reaches the neighbour's edge rows through
+    ghosts interpolated from the UNFILTERED field (quirk Q3), enters the eta
+    pass with FILTER_ETA's ghost indexing, and the whole FILTER_EXP equals the
+    composition built above from extended lines.  The ghosts come from
+    scalar_ghosts, whose routing is pinned by geometry above.  Ghosts taken from
+    the xi-filtered field instead move the receiver's edge rows by O(0.1)."""
+    n, nr = 16, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    # sender cells one to three rows/columns from an edge, away from corners:
+    # (panel, j, i) in 0-based indices, and whether the strip holding the cell
+    # feeds an eta-pass ghost (panel routing of derivative.f90:55-282:
+    # 6.N -> 1.S, 1.S -> 6.N, 5.W -> 4.N) or a xi-pass ghost (3.E -> 4.W,
+    # 4.S -> 6.W)
+    cells = [(5, n - 2, 7, True), (0, 2, 9, True), (4, 8, 1, True), (2, 10, n - 3, False), (3, 3, 6, False)]
+    for (p, j, i, feeds_eta) in cells:
+        f = np.zeros((6, nr, n, n))
+        f[p, :, j, i] = 1.0
+        gx, gy = o.scalar_ghosts(f)
+        want = _filter_exp_by_extension(f, gx, gy)
+        q = np.zeros((6, 5, nr, n, n))
+        q[:, 0] = f
+        got = o.filter_exp(q)[:, 0]
+        assert np.abs(got - want).max() < 1e-15, (p, j, i)
+        if not feeds_eta:
+            continue
+        # the counterfactual: ghosts of the xi-filtered field for the eta pass
+        t1 = np.zeros_like(f)
+        for pp in range(6):
+            ext = np.concatenate([gx[pp, 0][..., ::-1], f[pp], gx[pp, 1]], axis=2)
+            t1[pp] = _filter_line(ext, FILT, 2)
+        _, gy_filtered = o.scalar_ghosts(t1)
+        wrong = _filter_exp_by_extension(f, gx, gy_filtered)
+        assert np.abs(got - wrong).max() > 1e-2, (p, j, i)
+
+
+def test_filter_exp_treats_velocities_as_scalars():
+    """Quirk Q2 (filter.f90:25-28): v_xi and v_eta are interpolated as scalars.
+    An impulse in v_xi (or v_eta) filters exactly like the same impulse in rho',
+    stays in its own field on every panel, and never reaches the other velocity
+    component (a basis transform across panels 6 -> 1 would mix them)."""
+    n, nr = 16, 12
+    gr, c2, rho0, g0 = background(0, nr)
+    o = oracle.Oracle(n, nr, gr, c2, rho0, g0, 3)
+    for (p, j, i) in [(5, n - 2, 7), (4, 8, 1), (1, 2, 9)]:
+        q = np.zeros((6, 5, nr, n, n))
+        q[p, 0, :, j, i] = 1.0
+        ref = o.filter_exp(q)[:, 0]
+        for fld in (1, 2):
+            q = np.zeros((6, 5, nr, n, n))
+            q[p, fld, :, j, i] = 1.0
+            got = o.filter_exp(q)
+            assert np.array_equal(got[:, fld], ref), (p, fld)
+            for other in range(5):
+                if other != fld:
+                    assert not np.any(got[:, other]), (p, fld, other)
