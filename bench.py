@@ -1,11 +1,15 @@
 """Benchmark of the acoustic RK4 hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3|c2|c1|c5]
                     [--cpu-sample-steps S] [--no-cpu]
+    python bench.py --only copy     # plain device copy, for counter calibration
 
 A step is one RK4 call (timestep.f90:41-86): five fused stages plus the
 explicit filter over the whole six-panel grid, state resident in HBM.
 value = 6*n*n*nr*K / (wall seconds of K steps), grid-point RK4 steps/s.
+
+The headline workload is the 6x256x256x384 FP64 grid (c3); the line also
+carries the 6x128x128x256 grid (c2) as a nested object.
 
 Printed as one JSON line, with
   roofline     : the dominant kernel (the fused stage kernel), algorithmic
@@ -40,21 +44,47 @@ DESCR = {
 }
 
 
+def _cpu_threads():
+    """OpenMP threads of the CPU sample: OMP_NUM_THREADS when the host sets it
+    (the process's CPU share on a shared machine), else every CPU this process
+    may run on."""
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else avail
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    return threads, avail, os.cpu_count() or 1
+
+
 def _cpu_baseline(cfg, bg, q0, dt, sample_steps):
-    import numpy as np  # noqa: F401
-    cores = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    cores = int(os.environ["OMP_NUM_THREADS"])
+    threads, avail, nproc = _cpu_threads()
     import oracle
     o = oracle.Oracle(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
     o.rk4(q0, 1, dt)  # first touch / warm-up
     t = time.perf_counter()
     o.rk4(q0, sample_steps, dt)
     el = time.perf_counter() - t
-    return {"value": cfg.points * sample_steps / el, "unit": "grid-point RK4 steps/s", "cores": cores,
-            "kind": "port",
+    return {"value": cfg.points * sample_steps / el, "unit": "grid-point RK4 steps/s", "cores": threads,
+            "kind": "port", "nproc": nproc, "affinity_cpus": avail,
             "sample": f"{sample_steps} RK4 step(s) of the {cfg.name} workload ({cfg.points} points) "
-                      f"after one warm-up step, C restatement of the reference with OpenMP, {el:.1f} s"}
+                      f"after one warm-up step, C restatement of the reference with OpenMP on {threads} "
+                      f"threads (host: {nproc} CPUs, {avail} in this process's affinity), {el:.1f} s"}
+
+
+def _copy_calibration(reps=4, nbytes=1 << 30):
+    """A plain device copy of a known byte count (torch's copy of a 1 GiB FP64
+    buffer, reps times): the transfer against which the FETCH_SIZE and
+    WRITE_SIZE counters are calibrated (each copy reads and writes nbytes)."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    return {"copy": {"reps": reps, "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+                     "GB_per_s": 2 * nbytes * reps / el / 1e9}}
 
 
 def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
@@ -67,24 +97,14 @@ def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
     memory-sized segments and with checkpoints every 50 steps, and with FP32
     divergences; each gradient is checked by a central difference of J."""
     import numpy as np
-    from solar_amd import CONFIGS, Solver, make_inputs, panel_points
-    cfg = CONFIGS["c5"]
-    bg, q0, dt, _ = make_inputs(cfg)
-    gr, c2 = bg[0], bg[1]
+    from solar_amd import Solver, misfit_problem
+    prob = misfit_problem()
+    cfg, bg, q0, dt, rec, blob, c2_true = (prob[k] for k in ("cfg", "bg", "q0", "dt", "rec", "blob", "c2_true"))
+    c2 = bg[1]
     n, nr = cfg.n, cfg.nr
     steps = steps or cfg.steps
-    rng = np.random.default_rng(cfg.seed)
-    kr = int(np.argmin(np.abs(gr - 0.98)))
-    rec = np.array([[int(rng.integers(0, 6)), kr, int(rng.integers(0, n)), int(rng.integers(0, n))]
-                    for _ in range(16)])
-    xyz = panel_points(n)  # (6, n, n, 3) unit vectors
-    centre = rng.standard_normal(3)
-    centre /= np.linalg.norm(centre)
-    pos = gr[None, :, None, None, None] * xyz[:, None, :, :, :]
-    blob = np.exp(-np.sum((pos - 0.85 * centre) ** 2, axis=-1) / (2 * 0.08 ** 2))
-    c2_true = c2[None, :, None, None] * (1.0 + 0.02 * blob)
     out = {"workload": "c5: 6x96x96x192, stratified, 2000 RK4 steps, FP64", "steps": steps,
-           "receivers": int(rec.shape[0]), "receiver_radius": float(gr[kr])}
+           "receivers": int(rec.shape[0]), "receiver_radius": prob["receiver_radius"]}
     with Solver(n, nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
         s.set_receivers(rec)
         s.set_c2_field(c2_true)
@@ -112,7 +132,9 @@ def _adjoint_bench(fd_check, steps=None, checkpoint_every=0):
                          "phases_s": {k: round(v, 4) for k, v in s.adjoint_times().items()}}
             grads[name] = float(np.vdot(grad, dirn))
         out["J"] = J
-        out["ratio"] = out["fp64_div"]["ratio"]  # exact discrete gradient (the library default)
+        # the configuration as stated: FP64 divergences, checkpoints every 50 steps
+        out["ratio"] = out["fp64_div_ckpt50"]["ratio"]
+        out["ratio_memory_segments"] = out["fp64_div"]["ratio"]  # FP64, segments sized to free memory
         out["ratio_fp32_div"] = out["fp32_div"]["ratio"]  # stage divergences stored in FP32
         if fd_check:
             # central difference of J along the blob direction (step small enough that the
@@ -138,11 +160,12 @@ def _workload_descr(name, bits):
     return d if bits == 64 else d.replace("FP64", "FP32 state, FP64 arithmetic")
 
 
-def _c3_bench(steps, cpu, bits=64):
-    """The 6x256x256x384 grid: GPU rate, stage-kernel roofline, and a one-step
-    CPU sample of the same grid (after a warm-up step)."""
+def _grid_bench(name, steps, cpu_steps, bits=64):
+    """A second grid (by default the 6x128x128x256 one) in the same line: GPU
+    rate, stage-kernel roofline, and a CPU sample of cpu_steps steps of the
+    same grid (after a warm-up step); cpu_steps = 0 skips the CPU sample."""
     from solar_amd import CONFIGS, Solver, make_inputs
-    cfg = CONFIGS["c3"]
+    cfg = CONFIGS[name]
     bg, q0, dt, _ = make_inputs(cfg)
     with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits) as s:
         s.set_state(q0)
@@ -164,12 +187,12 @@ def _c3_bench(steps, cpu, bits=64):
     rate = cfg.points * steps / wall
     scale = bits / 64.0
     stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points * scale
-    out = {"workload": _workload_descr("c3", bits), "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
+    out = {"workload": _workload_descr(name, bits), "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
            "ms_per_step": wall / steps * 1e3, "step_frac": rate * STEP_BYTES * scale / HBM_PEAK,
            "stage_frac": stage_bytes / (kt["stage_ms"] * 1e-3) / HBM_PEAK, "state_finite": ok,
            "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
-    if cpu:
-        c = _cpu_baseline(cfg, bg, q0, dt, 1)
+    if cpu_steps:
+        c = _cpu_baseline(cfg, bg, q0, dt, cpu_steps)
         out["cpu_baseline"] = c
         out["gpu_over_cpu"] = rate / c["value"]
     return out
@@ -182,24 +205,14 @@ def _ensemble_bench(steps, reps):
     every step one hipGraph replay with one launch per kernel class and tile
     shape across all members, tiles chosen by the autotuner."""
     import numpy as np
-    from solar_amd import Config, Ensemble, background, generate_ic
-    rng = np.random.default_rng(20240)
-    spec = [(int(rng.choice([32, 48, 64, 96])), int(rng.choice([64, 128, 192])), float(rng.uniform(0.02, 0.08)),
-             int(rng.integers(1, 2**31))) for _ in range(48)]
-    members = []
-    for n, nr, width, seed in spec:
-        cfg = Config("ens", n, nr, 0, 0, steps)
-        gr, c2, rho0, g0 = background(0, nr)
-        members.append(dict(n=n, nr=nr, gr=gr, c2=c2, rho0=rho0, g0=g0, sponge_len=cfg.sponge_len,
-                            a_sponge=cfg.a_sponge, dt=cfg.dt(gr, c2)))
+    from solar_amd import Ensemble, ensemble_member_ic, ensemble_spec
+    members, spec = ensemble_spec(48, 20240)
     with Ensemble(members) as ens:
         t = time.perf_counter()
         ens.autotune(reps)
         tune_s = time.perf_counter() - t
-        for k, (n, nr, width, seed) in enumerate(spec):
-            q0, _ = generate_ic(0, n, nr, members[k]["gr"], members[k]["c2"], seed, [1e-3, width, 0.0])
-            ens.set_state(k, q0)
-            del q0
+        for k in range(len(members)):
+            ens.set_state(k, ensemble_member_ic(members[k], spec[k]))
         ens.rk4(3)
         ens.sync()
         t = time.perf_counter()
@@ -224,10 +237,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-sample-steps", type=int, default=4)
-    ap.add_argument("--no-c3", action="store_true",
-                    help="skip the 6x256x256x384 measurement (GPU rate, stage roofline, CPU sample)")
+    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--cpu-sample-steps", type=int, default=None,
+                    help="CPU sample steps of the headline grid (default: 1 for c3, 4 otherwise)")
+    ap.add_argument("--second", default="c2",
+                    help="second grid reported in a nested object ('none' to skip)")
+    ap.add_argument("--no-c3", action="store_true", help="same as --second none")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--state-bits", type=int, default=64, choices=(64, 32),
@@ -239,7 +254,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=None, help="steps of the adjoint workload (default 2000)")
     ap.add_argument("--checkpoint-every", type=int, default=0,
                     help="segment length of the adjoint run (0: the longest that fits in device memory)")
-    ap.add_argument("--only", choices=("adjoint", "ensemble"), default=None,
+    ap.add_argument("--only", choices=("adjoint", "ensemble", "copy"), default=None,
                     help="run one extra section alone (for profiling); prints its JSON object")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run STEPS steps in FP64 and FP32 storage and report the relative L2 error "
@@ -257,10 +272,14 @@ def main():
     if args.only == "ensemble":
         print(json.dumps({"ensemble": _ensemble_bench(args.ensemble_steps, 10)}), flush=True)
         return
+    if args.only == "copy":
+        print(json.dumps(_copy_calibration()), flush=True)
+        return
 
     from solar_amd import CONFIGS, Solver, make_inputs
     cfg = CONFIGS[args.workload]
-    steps = args.steps if args.steps is not None else cfg.steps
+    # default: the configuration's step count, at most 400 steps of timed work
+    steps = args.steps if args.steps is not None else min(cfg.steps, 400)
     bg, q0, dt, info = make_inputs(cfg)
 
     bits = args.state_bits
@@ -318,9 +337,9 @@ def main():
             ["rho", "v_xi", "v_eta", "v_r", "p"], errs))}
         del fin
 
-    c3 = None
-    if not args.no_c3 and args.workload != "c3":
-        c3 = _c3_bench(200, not args.no_cpu and bits == 64, bits)
+    second = None
+    if not args.no_c3 and args.second not in ("none", args.workload):
+        second = _grid_bench(args.second, 200, 0 if args.no_cpu or bits != 64 else 1, bits)
 
     ensemble = None
     if not args.no_ensemble and bits == 64:
@@ -332,7 +351,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = _cpu_baseline(cfg, bg, q0, dt, args.cpu_sample_steps)
+        cpu_steps = args.cpu_sample_steps or (1 if cfg.points > 100_000_000 else 4)
+        cpu = _cpu_baseline(cfg, bg, q0, dt, cpu_steps)
 
     out = {"metric": METRIC, "value": rate, "unit": "grid-point RK4 steps/s", "n_gpus": 1, "steps": steps,
            "warmup": args.warmup, "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -346,8 +366,8 @@ def main():
         out["adjoint"] = adjoint
     if ensemble:
         out["ensemble"] = ensemble
-    if c3:
-        out["c3"] = c3
+    if second:
+        out[args.second] = second
     if cpu:
         out["gpu_over_cpu"] = rate / cpu["value"]
     print(json.dumps(out), flush=True)

@@ -15,8 +15,8 @@ calls raise.
 """
 from .lib import (SolarError, Solver, Ensemble, background, generate_ic, host_tables, load_library,
                   panel_points, RHS_BC, RHS_SPONGE, ALPHA_RK4, BETA_RK4)
-from .config import CONFIGS, Config, make_inputs
+from .config import CONFIGS, Config, make_inputs, ensemble_spec, ensemble_member_ic, misfit_problem
 
 __all__ = ["SolarError", "Solver", "Ensemble", "background", "generate_ic", "host_tables", "load_library",
            "panel_points", "RHS_BC", "RHS_SPONGE", "ALPHA_RK4", "BETA_RK4", "CONFIGS", "Config",
-           "make_inputs"]
+           "make_inputs", "ensemble_spec", "ensemble_member_ic", "misfit_problem"]

@@ -60,3 +60,55 @@ def make_inputs(cfg):
     else:
         q0, info = generate_ic(1, cfg.n, cfg.nr, gr, c2, cfg.seed, [1e-3, 1e-6, 20.0])
     return (gr, c2, rho0, g0), q0, cfg.dt(gr, c2), info
+
+
+def ensemble_spec(count=48, seed=20240):
+    """The batched workload: count independent Gaussian-pulse simulations on
+    the uniform background, angular n from {32,48,64,96}, radial nr from
+    {64,128,192}, pulse widths in [0.02,0.08] and source positions from the
+    seed.  Returns (members, spec): members are the Ensemble() dicts, and
+    spec[m] = (n, nr, width, ic_seed) gives member m's initial state through
+    ensemble_member_ic."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    spec = [(int(rng.choice([32, 48, 64, 96])), int(rng.choice([64, 128, 192])), float(rng.uniform(0.02, 0.08)),
+             int(rng.integers(1, 2**31))) for _ in range(count)]
+    members = []
+    for n, nr, width, ic_seed in spec:
+        cfg = Config("ens", n, nr, 0, 0, 0)
+        gr, c2, rho0, g0 = background(0, nr)
+        members.append(dict(n=n, nr=nr, gr=gr, c2=c2, rho0=rho0, g0=g0, sponge_len=cfg.sponge_len,
+                            a_sponge=cfg.a_sponge, dt=cfg.dt(gr, c2)))
+    return members, spec
+
+
+def ensemble_member_ic(member, spec_entry):
+    n, nr, width, ic_seed = spec_entry
+    q0, _ = generate_ic(0, n, nr, member["gr"], member["c2"], ic_seed, [1e-3, width, 0.0])
+    return q0
+
+
+def misfit_problem(cfg=None, nrec=16, blob_amp=0.02, blob_width=0.08):
+    """The differentiation workload on a configuration (default c5,
+    6x96x96x192): nrec p' receivers at the radial point closest to r = 0.98
+    with seed-drawn panels and angular indices, and a "true" c^2 field carrying
+    a Gaussian blob of relative amplitude blob_amp at a seed-drawn position
+    (r = 0.85) from which the data are made.  Returns a dict with cfg, bg, q0,
+    dt, rec (nrec x (panel, k, j, i)), blob (6, nr, n, n) and c2_true."""
+    import numpy as np
+    from .lib import panel_points
+    cfg = cfg or CONFIGS["c5"]
+    bg, q0, dt, _ = make_inputs(cfg)
+    gr, c2 = bg[0], bg[1]
+    n = cfg.n
+    rng = np.random.default_rng(cfg.seed)
+    kr = int(np.argmin(np.abs(gr - 0.98)))
+    rec = np.array([[int(rng.integers(0, 6)), kr, int(rng.integers(0, n)), int(rng.integers(0, n))]
+                    for _ in range(nrec)])
+    xyz = panel_points(n)  # (6, n, n, 3) unit vectors
+    centre = rng.standard_normal(3)
+    centre /= np.linalg.norm(centre)
+    pos = gr[None, :, None, None, None] * xyz[:, None, :, :, :]
+    blob = np.exp(-np.sum((pos - 0.85 * centre) ** 2, axis=-1) / (2 * blob_width ** 2))
+    c2_true = c2[None, :, None, None] * (1.0 + blob_amp * blob)
+    return dict(cfg=cfg, bg=bg, q0=q0, dt=dt, rec=rec, blob=blob, c2_true=c2_true, receiver_radius=float(gr[kr]))

@@ -96,6 +96,19 @@ int dev_alloc(solar_ctx* c, T** p, size_t count) {
   return SOLAR_OK;
 }
 
+// Frees one buffer dev_alloc made (bytes long) and drops it from the handle's
+// list and byte count.  The caller has synchronised the handle's stream.
+void dev_free(solar_ctx* c, void* p, size_t bytes) {
+  if (!p) return;
+  for (size_t a = 0; a < c->allocs.size(); ++a)
+    if (c->allocs[a] == p) {
+      c->allocs.erase(c->allocs.begin() + static_cast<long>(a));
+      c->bytes -= bytes;
+      (void)hipFree(p);
+      return;
+    }
+}
+
 void release(solar_ctx* c) {
   if (c->adj_ws) (void)hipFree(c->adj_ws);
   c->adj_ws = nullptr;
@@ -687,6 +700,11 @@ int solar_set_receivers(solar_ctx* c, int nrec, const int* pkji) {
     idx[r] = ((static_cast<long long>(p) * 5 + 4) * c->nr + k) * c->n * c->n + static_cast<long long>(j) * c->n + i;
   }
   if (nrec > c->rec_cap) {
+    // the old, smaller buffer goes back before the larger one is taken
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dev_free(c, c->rec, static_cast<size_t>(c->rec_cap) * sizeof(long long));
+    c->rec = nullptr;
+    c->rec_cap = 0;
     long long* d = nullptr;
     int rc = dev_alloc(c, &d, static_cast<size_t>(nrec));
     if (rc) return rc;

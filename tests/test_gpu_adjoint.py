@@ -269,3 +269,19 @@ def test_tangent_linear_dot_product():
                 rel = np.linalg.norm(fd - dseis) / np.linalg.norm(dseis)
                 print(f"[adj] TL traces vs central differences: rel L2 {rel:.2e}", flush=True)
                 assert rel < 1e-5
+
+
+def test_receiver_buffer_replaced_not_leaked():
+    """Growing receiver sets replace the device buffer: the handle's device
+    bytes are its base plus the largest set so far (8 B per receiver)."""
+    cfg, bg, dt = _setup(16, 24, 1)
+    rng = np.random.default_rng(9)
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge) as s:
+        base = s.device_bytes()
+        for count, cap in ((4, 4), (8, 8), (16, 16), (2, 16), (16, 16)):
+            rec = np.stack([rng.integers(0, 6, count), rng.integers(0, cfg.nr, count),
+                            rng.integers(0, cfg.n, count), rng.integers(0, cfg.n, count)], axis=1)
+            s.set_receivers(rec)
+            assert s.device_bytes() == base + 8 * cap, (count, s.device_bytes() - base)
+        s.set_state(np.zeros((6, 5, cfg.nr, cfg.n, cfg.n)))
+        assert s.record(2, dt).shape == (2, 16)

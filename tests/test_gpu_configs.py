@@ -1,0 +1,192 @@
+"""The five workload configurations at their full sizes, on the GPU.
+
+  c2  6x128x128x256 stratified FP64: steps bit-identical to the CPU restatement
+  c3  6x256x256x384 stratified: one FP64 step bit-identical to the CPU
+      restatement; the FP32-state run against the FP64 run, per field
+  c4  the 48-member uneven ensemble (autotuned tiles): every member bit-equal
+      to its single-grid run, and the smallest one to the CPU restatement
+  c5  6x96x96x192 misfit gradient with 16 receivers and checkpoints every 50
+      steps over a shortened run: central differences along 8 random
+      directions, the tangent-linear dot product, and the same bits for any
+      checkpoint spacing
+
+The CPU restatement runs with OpenMP on the host's cores: about 2.5 s per c2
+step and 16 s per c3 step on 16 threads.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from solar_amd import (CONFIGS, Ensemble, Solver, ensemble_member_ic, ensemble_spec, make_inputs,
+                       misfit_problem)
+from tests.conftest import per_field_rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12  # FP64 relative L2 per field
+
+
+def _parity(got, want, what):
+    errs = per_field_rel_l2(got, want)
+    ndiff = int(np.count_nonzero(got != want))
+    print(f"[parity] {what}: differing={ndiff} of {got.size} rel_l2={['%.2e' % e for e in errs]}", flush=True)
+    assert all(e <= TOL for e in errs), (what, errs)
+    assert ndiff == 0, what
+
+
+def _gpu_run(cfg, bg, q0, dt, steps, bits=64):
+    with Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge, state_bits=bits) as s:
+        s.set_state(q0)
+        s.rk4(steps, dt)
+        return s.get_state()
+
+
+def test_c2_full_grid_steps_bit_exact():
+    """c2 at 6x128x128x256 (stratified background, sponge, spherical-harmonic
+    mode plus noise): 2 RK4 steps equal the CPU restatement bit for bit."""
+    cfg = CONFIGS["c2"]
+    bg, q0, dt, _ = make_inputs(cfg)
+    print(f"[case] c2 n={cfg.n} nr={cfg.nr} dt={dt:.6g} steps=2", flush=True)
+    got = _gpu_run(cfg, bg, q0, dt, 2)
+    t = time.perf_counter()
+    want, _ = oracle.Oracle(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge).rk4(q0, 2, dt)
+    print(f"[c2] CPU restatement {time.perf_counter() - t:.1f} s", flush=True)
+    assert np.abs(want[:, 4]).max() > 1e-6
+    _parity(got, want, "c2 x2")
+
+
+@pytest.fixture(scope="module")
+def c3_inputs():
+    cfg = CONFIGS["c3"]
+    bg, q0, dt, _ = make_inputs(cfg)
+    return cfg, bg, q0, dt
+
+
+def test_c3_full_grid_step_bit_exact(c3_inputs):
+    """c3 at 6x256x256x384 FP64: one RK4 step equals the CPU restatement bit
+    for bit (151M points, the grid the throughput target is quoted on)."""
+    cfg, bg, q0, dt = c3_inputs
+    print(f"[case] c3 n={cfg.n} nr={cfg.nr} dt={dt:.6g} steps=1", flush=True)
+    got = _gpu_run(cfg, bg, q0, dt, 1)
+    t = time.perf_counter()
+    want, _ = oracle.Oracle(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge).rk4(q0, 1, dt)
+    print(f"[c3] CPU restatement {time.perf_counter() - t:.1f} s", flush=True)
+    _parity(got, want, "c3 x1")
+
+
+C3_FP32_STEPS = 200
+# Relative L2 per field of the FP32-state run against the FP64 run after 200
+# c3 steps (rho', v_xi, v_eta, v_r, p'): the float rounding of the stored state
+# accumulates to a few 1e-7 in the velocities and ~1e-6 in p' over 2000 steps.
+C3_FP32_BOUND = (2e-6, 2e-6, 2e-6, 2e-6, 1e-5)
+
+
+def test_c3_fp32_state_against_fp64(c3_inputs):
+    cfg, bg, q0, dt = c3_inputs
+    print(f"[case] c3 FP32 state vs FP64, {C3_FP32_STEPS} steps", flush=True)
+    f64 = _gpu_run(cfg, bg, q0, dt, C3_FP32_STEPS, 64)
+    f32 = _gpu_run(cfg, bg, q0, dt, C3_FP32_STEPS, 32)
+    errs = per_field_rel_l2(f32, f64)
+    print(f"[fp32] c3 x{C3_FP32_STEPS}: rel_l2 per field {['%.2e' % e for e in errs]}", flush=True)
+    assert all(np.isfinite(f32).ravel()[::997])
+    for e, b in zip(errs, C3_FP32_BOUND):
+        assert e < b, (errs, C3_FP32_BOUND)
+    assert errs[4] > 1e-10  # the run really stored FP32
+
+
+def test_c4_ensemble_members_bit_equal_single_grid_runs():
+    """The 48-member spec (n in {32,48,64,96}, nr in {64,128,192}), tiles from
+    the autotuner, 3 steps: each member equals its single-grid run bit for bit,
+    and the smallest member equals the CPU restatement."""
+    members, spec = ensemble_spec(48, 20240)
+    steps = 3
+    with Ensemble(members) as ens:
+        ens.autotune(2)
+        info = ens.info()
+        print(f"[case] c4 {len(members)} members, {info['points']} points, "
+              f"{info['launches_per_step']} launches/step", flush=True)
+        for k in range(len(members)):
+            ens.set_state(k, ensemble_member_ic(members[k], spec[k]))
+        ens.rk4(steps)
+        for k, m in enumerate(members):
+            got = ens.get_state(k)
+            with Solver(m["n"], m["nr"], m["gr"], m["c2"], m["rho0"], m["g0"], m["sponge_len"],
+                        m["a_sponge"]) as s:
+                s.set_state(ensemble_member_ic(m, spec[k]))
+                s.rk4(steps, m["dt"])
+                want = s.get_state()
+            assert np.array_equal(got, want), f"member {k} (n={m['n']}, nr={m['nr']}, tiles={ens.tiles(k)})"
+            if k == min(range(len(members)), key=lambda j: spec[j][0] ** 2 * spec[j][1]):
+                ref, _ = oracle.Oracle(m["n"], m["nr"], m["gr"], m["c2"], m["rho0"], m["g0"], m["sponge_len"],
+                                       m["a_sponge"]).rk4(ensemble_member_ic(m, spec[k]), steps, m["dt"])
+                assert np.array_equal(got, ref), f"member {k} against the CPU restatement"
+                print(f"[c4] member {k} (n={m['n']}, nr={m['nr']}) equals the CPU restatement", flush=True)
+
+
+C5_STEPS = 150  # shortened from 2000; three segments at checkpoints every 50 steps
+
+
+@pytest.fixture(scope="module")
+def c5():
+    prob = misfit_problem()
+    cfg, bg = prob["cfg"], prob["bg"]
+    print(f"[case] c5 n={cfg.n} nr={cfg.nr} receivers={len(prob['rec'])} steps={C5_STEPS}", flush=True)
+    s = Solver(cfg.n, cfg.nr, *bg, cfg.sponge_len, cfg.a_sponge)
+    s.set_receivers(prob["rec"])
+    s.set_c2_field(prob["c2_true"])
+    s.set_state(prob["q0"])
+    prob["data"] = s.record(C5_STEPS, prob["dt"])
+    prob["c2_0"] = np.broadcast_to(bg[1][None, :, None, None], prob["c2_true"].shape).copy()
+    s.set_c2_field(prob["c2_0"])
+    J, grad, seis = s.misfit_gradient(prob["q0"], C5_STEPS, prob["dt"], prob["data"], 50)
+    prob.update(J=J, grad=grad, seis=seis, segment=s.segment_used)
+    yield s, prob
+    s.close()
+
+
+def test_c5_gradient_against_central_differences(c5):
+    s, p = c5
+    q0, dt, data, c2_0, grad = p["q0"], p["dt"], p["data"], p["c2_0"], p["grad"]
+    assert p["segment"] == 50 and p["J"] > 0
+
+    def J(c2):
+        s.set_c2_field(c2)
+        s.set_state(q0)
+        r = s.record(C5_STEPS, dt) - data
+        return float(np.sum(r * r))
+
+    assert abs(J(c2_0) - p["J"]) <= 1e-13 * p["J"]
+    rng = np.random.default_rng(50)
+    errs = []
+    for t in range(8):
+        d = rng.standard_normal(c2_0.shape) * c2_0
+        eps = 1e-5
+        fd = (J(c2_0 + eps * d) - J(c2_0 - eps * d)) / (2 * eps)
+        ad = float(np.vdot(grad, d))
+        errs.append(abs(fd - ad) / abs(fd))
+        print(f"[c5] dir {t}: fd={fd:.12e} adj={ad:.12e} rel={errs[-1]:.2e}", flush=True)
+    s.set_c2_field(c2_0)
+    assert max(errs) < 1e-6, errs
+
+
+def test_c5_tangent_linear_dot_product(c5):
+    s, p = c5
+    rng = np.random.default_rng(51)
+    dc2 = rng.standard_normal(p["c2_0"].shape) * p["c2_0"]
+    s.set_c2_field(p["c2_0"])
+    dseis = s.tangent_linear(p["q0"], C5_STEPS, p["dt"], dc2)
+    lhs = float(np.sum(2.0 * (p["seis"] - p["data"]) * dseis))
+    rhs = float(np.vdot(p["grad"], dc2))
+    print(f"[c5] TL dot: {lhs:.15e} vs {rhs:.15e} rel={abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+def test_c5_gradient_independent_of_checkpoint_spacing(c5):
+    s, p = c5
+    s.set_c2_field(p["c2_0"])
+    for every in (0, 37):
+        _, g, _ = s.misfit_gradient(p["q0"], C5_STEPS, p["dt"], p["data"], every)
+        print(f"[c5] checkpoint_every={every}: segment {s.segment_used}", flush=True)
+        assert np.array_equal(g, p["grad"]), every
