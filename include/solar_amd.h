@@ -48,7 +48,8 @@ enum {
  * (interpolation.f90:9), the partner-rank tables of INITIALIZE_MPI
  * (init_mpi.f90:164-304) and the background arrays SOLAR_DATA fills
  * (ic_acoustic_wave.f90:94-108, radial-only there, so passed as profiles).
- *   n          angular points per panel edge (nx = ny), multiple of 16
+ *   n          angular points per panel edge (nx = ny), 16 <= n <= 512 (any n;
+ *              the sensitivity entry points and ensembles need a multiple of 16)
  *   nr         radial points, >= 12
  *   gr, c2, rho0, g0   radial profiles, length nr (nondimensional)
  *   sponge_len, a_sponge   sponge.f90:27,20 (k_sponge_length, a_sponge)

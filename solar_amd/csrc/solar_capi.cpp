@@ -612,6 +612,10 @@ int solar_host_tables(int n, int nr, const double* gr, const double* c2, const d
 // ------------------------------------------------------------ sensitivities
 
 static int adj_alloc(solar_ctx* c) {
+  // the adjoint kernels tile the panel exactly (16x16 or 32x8 tiles)
+  if (c->n % 16 != 0)
+    return fail(SOLAR_ERR_ARG, "adjoint: needs a panel size n that is a multiple of 16 (n = " + std::to_string(c->n) +
+                                   "); the forward path takes any n");
   if (c->lam) return SOLAR_OK;
   if (c->f32) return fail(SOLAR_ERR_ARG, "adjoint: needs a 64-bit state handle");
   const size_t nq = c->nf * 30;

@@ -210,6 +210,7 @@ struct StagePlane {
   // flight); the double members then hold the ghost-strip cells only
   S fpx, fvx, fpy0, fve0, fpy1, fve1;
   S pn, rn;                    // p', v_r of the plane entering the window
+  double ogp, ogv;             // PART: ghosts of an outside thread's own cell (p' and v_xi or v_eta)
   double c2;                   // optional 3-D c^2 at (i,j,k)
   double tdc2, tdiv;           // tangent-linear inputs at (i,j,k)
 };
@@ -251,7 +252,13 @@ struct StagePlane {
 //   the edge columns i = 1..3 and n-4..n-2 in the edge_cols layout, for the
 //   next gather -- p', v_xi, v_eta after stages 1-4 (the vector gather), all
 //   five fields after the last stage (the filter gather).
-template <int TI, int TJ, int KIND, typename S, int MODE = 0>
+//   PART: the tiles need not divide n.  The last tile of a row or column
+//   then has threads past the panel edge ("outside" threads): they store
+//   nothing, and those within three points of the edge bring the ghost values
+//   of their own cell (E ghosts of p' and v_xi, N ghosts of p' and v_eta)
+//   into the LDS tile, where the in-panel threads' stencils read them.  Every
+//   other cell an outside thread would load reads a fixed in-bounds address.
+template <int TI, int TJ, int KIND, typename S, int MODE = 0, bool PART = false>
 __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Targ, const S* __restrict__ qin_a,
                                                    S* __restrict__ qout_a, S* __restrict__ qw_a,
                                                    const double* __restrict__ gxp_a, const double* __restrict__ gyp_a,
@@ -279,7 +286,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     k0 = w.z >> 8;
     k1 = w.w;
   } else {
-    const int nti = Targ.n / TI, ntj = Targ.n / TJ;
+    const int nti = (Targ.n + TI - 1) / TI, ntj = (Targ.n + TJ - 1) / TJ;
     const int nb = nti * ntj * 6 * nkch;
     int b = xcd_remap(blockIdx.x, nb);
     bi = b % nti;
@@ -305,9 +312,13 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
+  // own point inside the panel (always, unless PART); outside threads load
+  // through the nearest in-panel point and store nothing
+  const bool own_in = !PART || (i < n && j < n);
+  const int ic = PART ? min(i, n - 1) : i, jc = PART ? min(j, n - 1) : j;
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
-  const size_t ij = (size_t)j * n + i;
+  const size_t ij = (size_t)jc * n + ic;
   const auto f0 = qin + (size_t)(P * 5) * nf;  // rho'
   const auto f1 = f0 + nf;                     // v_xi
   const auto f2 = f1 + nf;                     // v_eta
@@ -317,7 +328,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const auto w0 = qw + (size_t)(P * 5) * nf;
   // edge-column copy: layer l = i (W) or n-1-i (E); the others store none
   const int ecl = i <= 3 ? i : n - 1 - i;
-  const bool ec_on = !ENS && !rhs_mode && ecols_a != nullptr && ecl >= 1 && ecl <= 3;
+  const bool ec_on = !ENS && !rhs_mode && ecols_a != nullptr && ecl >= 1 && ecl <= 3 && own_in;
   const auto ecw = to_global(ec_on ? ecols_a : qout_a);  // qout_a: never stored through when ec_on is false
   const size_t ec_base = ec_on ? (size_t)(P * 2 + (i > 3)) * nr * edge_cols_plane(n) + (size_t)j * 3 + (ecl - 1) : 0;
 
@@ -335,11 +346,15 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   int hx_off = 0, hy0_off = 0, hy1_off = 0;
   bool hx_g = false, hy0_g = false, hy1_g = false;
   int hx_col = 0;
+  // PART: a slot whose cell lies beyond the panel and beyond its three ghost
+  // layers, or in a row or column outside the panel, keeps the fixed address
+  // (no stencil of an in-panel point reads it).
   if (hx) {
     const bool lo = tx < 3;
     const int ii = lo ? i0 - 3 + tx : i + 3;
     hx_col = lo ? tx : tx + 6;
-    if (ii >= 0 && ii < n) {
+    if (PART && (j >= n || ii >= n + 3)) {
+    } else if (ii >= 0 && ii < n) {
       hx_off = j * n + ii;
     } else {
       hx_g = true;
@@ -348,7 +363,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   }
   if (hy0) {
     const int jj = j0 - 3 + ty;
-    if (jj >= 0) {
+    if (PART && i >= n) {
+    } else if (jj >= 0) {
       hy0_off = jj * n + i;
     } else {
       hy0_g = true;
@@ -357,11 +373,30 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   }
   if (hy1) {
     const int jj = j + 3;
-    if (jj < n) {
+    if (PART && (i >= n || jj >= n + 3)) {
+    } else if (jj < n) {
       hy1_off = jj * n + i;
     } else {
       hy1_g = true;
       hy1_off = (int)gy_side + (jj - n) * n + i;
+    }
+  }
+  // PART: an outside thread within three points of the E edge (in a panel
+  // row) or of the N edge (in a panel column) owns an LDS cell that holds
+  // ghosts: E ghosts of p' and v_xi, or N ghosts of p' and v_eta.
+  int og = 0, og_off = 0;
+  auto ogp_src = gxpW, ogv_src = gxvW;  // without ghosts: a fixed in-bounds address
+  if (PART) {
+    if (j < n && i >= n && i < n + 3) {
+      og = 1;
+      ogp_src = gxpW + gx_side;
+      ogv_src = gxvW + gx_side;
+      og_off = j * 3 + (i - n);
+    } else if (i < n && j >= n && j < n + 3) {
+      og = 2;
+      ogp_src = gypS + gy_side;
+      ogv_src = gyvS + gy_side;
+      og_off = (j - n) * n + i;
     }
   }
 
@@ -414,6 +449,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
         L.hve1 = gyvS[kg + gy];
       }
     }
+    if constexpr (PART) {
+      L.ogp = ogp_src[kg + og_off];
+      L.ogv = ogv_src[kg + og_off];
+    }
     // plane entering the radial window when the window slides before k
     const int kn = min(k + 3, nr - 1);
     L.pn = f4[(size_t)kn * plane + ij];
@@ -428,7 +467,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
   };
 
-  const double Ci = to_global(T.ci)[i], Dj = to_global(T.dj)[j];
+  const double Ci = to_global(T.ci)[ic], Dj = to_global(T.dj)[jc];
   const double dc = to_global(T.dc)[ij], tdx = to_global(T.tdx)[ij], tde = to_global(T.tde)[ij],
                xy = to_global(T.xy)[ij];
 
@@ -454,7 +493,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   int pk = 0;  // the plane of px
   auto flush = [&]() {
     const S pq[5] = {pq0, pq1, pq2, pq3, pq4}, pwv[5] = {pw0, pw1, pw2, pw3, pw4};
-    if (rhs_mode) {
+    if (!own_in) {
+    } else if (rhs_mode) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) o0[f * nf + px] = pq[f];
     } else {
@@ -497,9 +537,19 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
     const double pc = pcs, rc = rcs;
     const int bf = k & 1;
-    sp[bf][ty + 3][tx + 3] = pc;
-    svx[bf][ty][tx + 3] = cur.vx;
-    sve[bf][ty + 3][tx] = cur.ve;
+    double own_p = pc, own_vx = cur.vx, own_ve = cur.ve;
+    if constexpr (PART) {
+      if (og == 1) {
+        own_p = cur.ogp;
+        own_vx = cur.ogv;
+      } else if (og == 2) {
+        own_p = cur.ogp;
+        own_ve = cur.ogv;
+      }
+    }
+    sp[bf][ty + 3][tx + 3] = own_p;
+    svx[bf][ty][tx + 3] = own_vx;
+    sve[bf][ty + 3][tx] = own_ve;
     if (hx) {
       sp[bf][ty + 3][hx_col] = kWide || hx_g ? cur.hpx : static_cast<double>(cur.fpx);
       svx[bf][ty][hx_col] = kWide || hx_g ? cur.hvx : static_cast<double>(cur.fvx);
@@ -548,7 +598,7 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
     const double c2rho0 = c2f ? cur.c2 * row.rho0 : row.c2rho0;
     const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
-    if (divout) {  // div(v) of the stage input, kept for the adjoint (FP64, or FP32 when div32)
+    if (divout && own_in) {  // div(v) of the stage input, kept for the adjoint (FP64, or FP32 when div32)
       if (div32)
         reinterpret_cast<float*>(divout)[dx] = static_cast<float>(div);
       else
@@ -615,7 +665,9 @@ struct FilterLds {
 };
 
 // The tile (bi, bj) of field F of panel P (PF = 5P + F) over planes [k0, k1).
-template <int TI, int TJ, typename S>
+// PART: the tile may reach past the panel edge; threads past it store nothing,
+// and tile cells beyond the panel's three ghost layers stay 0.
+template <int TI, int TJ, typename S, bool PART = false>
 __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, int PF, int k0, int k1,
                                             const S* __restrict__ qin, S* __restrict__ qout,
                                             const double* __restrict__ fgx, const double* __restrict__ fgy,
@@ -630,6 +682,7 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int i0 = bi * TI, j0 = bj * TJ;
   const int i = i0 + tx, j = j0 + ty;
+  const bool own_in = !PART || (i < n && j < n);
 
   const size_t plane = (size_t)n * n, nf = plane * nr;
   const S* g = qin + (size_t)PF * nf;
@@ -638,11 +691,12 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   const size_t fstride = (size_t)12 * gx_side;
   const double* gxW = fgx + F * fstride + (size_t)(P * 2) * gx_side;
   const double* gyS = fgy + F * fstride + (size_t)(P * 2) * gy_side;
-  const size_t ij = (size_t)j * n + i;
+  const size_t ij = own_in ? (size_t)j * n + i : 0;
 
   // filter.f90:652,726-777: the five planes at each end are zero.
-  for (int k = k0; k < k1; ++k)
-    if (k < kNstr || k >= nr - kNstr) o[(size_t)k * plane + ij] = static_cast<S>(0.0);
+  if (own_in)
+    for (int k = k0; k < k1; ++k)
+      if (k < kNstr || k >= nr - kNstr) o[(size_t)k * plane + ij] = static_cast<S>(0.0);
   const int a = max(k0, kNstr), bnd = min(k1, nr - kNstr);
   if (a >= bnd) return;
 
@@ -674,11 +728,11 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
         fp[c] = g + (size_t)gj * n + gi;
         fst[c] = (unsigned)plane;
         kind[c] = 1;
-      } else if (jin) {
+      } else if (jin && gi < n + 3) {  // gi >= -3 always; beyond the E ghosts: none
         gp[c] = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
         gst[c] = 3u * n;
         kind[c] = 2;
-      } else if (iin) {
+      } else if (iin && gj < n + 3) {
         gp[c] = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
         gst[c] = 3u * n;
         kind[c] = 2;
@@ -773,7 +827,7 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
       double s = 0.0;
 #pragma unroll
       for (int q = 0; q < 11; ++q) s = s + T.filtr[q] * win[q];
-      o[(size_t)(kk - kNstr) * plane + ij] = static_cast<S>(s);
+      if (own_in) o[(size_t)(kk - kNstr) * plane + ij] = static_cast<S>(s);
     }
   };
   // Branch-free memory traffic per iteration (a fixed warm-up, then pairs of
@@ -796,13 +850,13 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   if (kk < kend) step(kk, pa, Store());
 }
 
-template <int TI, int TJ, typename S>
+template <int TI, int TJ, typename S, bool PART = false>
 __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restrict__ qin,
                                                     S* __restrict__ qout, const double* __restrict__ fgx,
                                                     const double* __restrict__ fgy, int kchunk, int nkch) {
   __shared__ FilterLds<TI, TJ> lds;
   const int n = T.n, nr = T.nr;
-  const int nti = n / TI, ntj = n / TJ;
+  const int nti = (n + TI - 1) / TI, ntj = (n + TJ - 1) / TJ;
   const int nb = nti * ntj * 30 * nkch;
   int b = xcd_remap(blockIdx.x, nb);
   const int bi = b % nti;
@@ -814,7 +868,7 @@ __global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restr
   const int k0 = ch * kchunk;
   const int k1 = min(k0 + kchunk, nr);
   if (k0 >= nr) return;
-  filter_tile<TI, TJ, S>(T, bi, bj, PF, k0, k1, qin, qout, fgx, fgy, lds);
+  filter_tile<TI, TJ, S, PART>(T, bi, bj, PF, k0, k1, qin, qout, fgx, fgy, lds);
 }
 
 // Ensemble FILTER_EXP: work items (member, tile, 5P+F, plane range), q[cur] -> q[1-cur].
@@ -950,23 +1004,23 @@ hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, 
                          gxs, gys, gxv, gyv, st);
 }
 
-template <int TI, int TJ, typename S>
+template <int TI, int TJ, typename S, bool PART = false>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
                                  double* divout, int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
-  const int tiles = (T.n / TI) * (T.n / TJ) * 6;
+  const int tiles = ((T.n + TI - 1) / TI) * ((T.n + TJ - 1) / TJ) * 6;
   // radial chunks of >= 32 planes, ~12k blocks: measured best at 6x128^2x256 and 6x256^2x384
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 12288, 32);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   const int kind = (rhs_mode || stage == 0) ? 0 : (stage < 4 ? 1 : 2);
-#define SOLAR_STAGE_LAUNCH(K, M)                                                                                \
-  hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, be, \
-                     dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, ecols, nullptr, nullptr, 0, 0)
+#define SOLAR_STAGE_LAUNCH(K, M)                                                                                  \
+  hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M, PART>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, \
+                     be, dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, ecols, nullptr, nullptr, 0, 0)
   if (tl_dc2) {
-    if constexpr (sizeof(S) == 8) {
+    if constexpr (sizeof(S) == 8 && !PART) {
       if (kind == 0)
         SOLAR_STAGE_LAUNCH(0, 2);
       else if (kind == 1)
@@ -974,7 +1028,7 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
       else
         SOLAR_STAGE_LAUNCH(2, 2);
     } else {
-      return hipErrorInvalidValue;  // the tangent-linear stage is FP64 only
+      return hipErrorInvalidValue;  // the tangent-linear stage is FP64 only, on tiles that divide n
     }
   } else if (kind == 0) {
     SOLAR_STAGE_LAUNCH(0, 0);
@@ -989,14 +1043,24 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
 
 // Tile shapes (TI x TJ) compiled for the stage and filter kernels; the
 // default is the measured best for the widest TI dividing n.  A requested
-// shape that does not divide n falls back to the default.
+// shape that does not divide n falls back to the default.  When no compiled
+// shape divides n (n not a multiple of 16), the partial-tile kernels run with
+// 32-wide or 16-wide tiles, whichever pads n less (ties to 32).
 static bool fits(int n, int ti, int tj) { return ti > 0 && tj > 0 && n % ti == 0 && n % tj == 0; }
+static int pad_pick_ti(int n) { return ((n + 31) / 32) * 32 <= ((n + 15) / 16) * 16 ? 32 : 16; }
 
 template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                                  hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
                                  int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
+  if (T.n % 16 != 0) {  // partial tiles: 32x8 or 16x16
+    if (pad_pick_ti(T.n) == 32)
+      return launch_stage_t<32, 8, S, true>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f,
+                                            divout, div32, tl_dc2, tl_div, ecols);
+    return launch_stage_t<16, 16, S, true>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f,
+                                           divout, div32, tl_dc2, tl_div, ecols);
+  }
   int ti = ls.stage_ti, tj = ls.stage_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 32x8 (64x8 within 2%), then 16x16
     ti = pick_ti(T.n);
@@ -1037,22 +1101,25 @@ hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* q
                         static_cast<double*>(ecols));
 }
 
-template <int TI, int TJ, typename S>
+template <int TI, int TJ, typename S, bool PART = false>
 static hipError_t launch_filter_t(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
                                   hipStream_t st, const LaunchShape& ls) {
-  const int tiles = (T.n / TI) * (T.n / TJ) * 30;
+  const int tiles = ((T.n + TI - 1) / TI) * ((T.n + TJ - 1) / TJ) * 30;
   // radial chunks of >= 64 planes (each re-reads 10 halo planes), ~4k blocks: a single chunk per
   // column at 6x256^2x384 leaves a long tail (3.45 ms against 2.96 ms with two)
   const int nkch = ls.filter_chunks > 0 ? ls.filter_chunks : chunks_for(tiles, T.nr, 4096, 64);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
-  hipLaunchKernelGGL((k_filter<TI, TJ, S>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
+  hipLaunchKernelGGL((k_filter<TI, TJ, S, PART>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
   return hipGetLastError();
 }
 
 template <typename S>
 static hipError_t launch_filter_s(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
                                   hipStream_t st, const LaunchShape& ls) {
+  if (T.n % 16 != 0)  // partial tiles
+    return pad_pick_ti(T.n) == 32 ? launch_filter_t<32, 16, S, true>(T, qin, qout, fgx, fgy, st, ls)
+                                  : launch_filter_t<16, 16, S, true>(T, qin, qout, fgx, fgy, st, ls);
   int ti = ls.filter_ti, tj = ls.filter_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 32x16 (n%32==0), 16x16
     ti = T.n % 32 == 0 ? 32 : 16;

@@ -63,8 +63,8 @@ const Route kRoutes[6][4] = {
 
 bool build_tables(int n, int nr, const double* gr, const double* c2, const double* rho0, const double* g0,
                   int sponge_len, double a_sponge, Tables* t, std::string* err) {
-  if (n < 16 || n % 16 != 0 || n > 512) {
-    if (err) *err = "panel size n must be a multiple of 16 in [16, 512] (got " + std::to_string(n) + ")";
+  if (n < 16 || n > 512) {
+    if (err) *err = "panel size n must lie in [16, 512] (got " + std::to_string(n) + ")";
     return false;
   }
   if (nr < 12) {

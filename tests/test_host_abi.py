@@ -27,7 +27,7 @@ def test_library_exports_every_declared_symbol():
     assert names == {e[0] for e in EXPORTS}
 
 
-@pytest.mark.parametrize("n,nr,kind", [(16, 24, 1), (32, 64, 0), (48, 20, 1), (96, 40, 1)])
+@pytest.mark.parametrize("n,nr,kind", [(16, 24, 1), (32, 64, 0), (48, 20, 1), (96, 40, 1), (20, 24, 1), (100, 30, 1)])
 def test_host_tables_equal_oracle_tables(n, nr, kind):
     gr, c2, rho0, g0 = background(kind, nr)
     L = max(1, round(0.08 * nr))
@@ -44,8 +44,11 @@ def test_host_tables_equal_oracle_tables(n, nr, kind):
 
 def test_invalid_sizes_rejected():
     gr, c2, rho0, g0 = background(0, 24)
-    with pytest.raises(solar_amd.SolarError, match="multiple of 16"):
-        host_tables(20, 24, gr, c2, rho0, g0, 2, 0.1)
+    for n in (15, 513):
+        with pytest.raises(solar_amd.SolarError, match=r"\[16, 512\]"):
+            host_tables(n, 24, gr, c2, rho0, g0, 2, 0.1)
+    with pytest.raises(solar_amd.SolarError, match="at least 12"):
+        host_tables(16, 11, gr[:11], c2[:11], rho0[:11], g0[:11], 2, 0.1)
     with pytest.raises(solar_amd.SolarError, match="sponge"):
         host_tables(16, 24, gr, c2, rho0, g0, 20, 0.1)
 
