@@ -99,6 +99,20 @@ int solar_sync(solar_ctx* ctx);
  * arrays q(n,n,nr,5,6) -> q_rhs(n,n,nr,5,6).  Does not touch the state. */
 int solar_compute_rhs(solar_ctx* ctx, const double* q, double* q_rhs, int flags);
 
+/* solar_compute_rhs on device arrays, allocating nothing: no hipMalloc,
+ * hipFree or other allocating call.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  q and q_rhs are device arrays
+ * q(n,n,nr,5,6) in the handle's storage type (q_rhs must not overlap q);
+ * flags as solar_compute_rhs.  The ghost strips of the panel exchange live in
+ * a static device array of the library (128 MiB, held by every GPU the
+ * library runs on until the process ends), so n*nr may be at most 116508
+ * (6x256x256x384 fits); larger grids return SOLAR_ERR_ARG.  Enqueued on
+ * `stream` (a hipStream_t; NULL = the handle's stream) without waiting.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (other streams, other threads, other handles) race. */
+int solar_compute_rhs_noalloc(solar_ctx* ctx, const void* q, void* q_rhs, int flags, void* stream);
+
 /* FILTER_EXP(q, qf) (filter.f90:9-48) on host arrays; does not touch the state. */
 int solar_filter_exp(solar_ctx* ctx, const double* q, double* qf);
 

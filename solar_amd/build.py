@@ -19,7 +19,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I/opt/rocm/include"]
-SOURCES_HIP = ["solar_kernels.hip", "solar_adjoint.hip"]
+SOURCES_HIP = ["solar_kernels.hip", "solar_adjoint.hip", "solar_noalloc.hip"]
 SOURCES_HOST = ["solar_tables.cpp", "solar_ic.cpp", "solar_capi.cpp", "solar_ensemble.cpp", "solar_io.cpp"]
 
 

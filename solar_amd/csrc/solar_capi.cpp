@@ -79,6 +79,9 @@ struct solar_ctx {
   size_t adj_ws_bytes = 0;
   double adj_times[4] = {0, 0, 0, 0};  // last gradient: workspace, forward, recompute, adjoint (s)
   int nrec = 0, rec_cap = 0;
+  // static device scratch of solar_compute_rhs_noalloc on this handle's device
+  void* scratch = nullptr;
+  hipError_t scratch_status = hipSuccess;
 };
 
 namespace {
@@ -294,6 +297,9 @@ int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double
     c->ecols = ec;
   }
   if (hipMemset(c->q[0], 0, nq * c->esz) != hipSuccess) return bail(fail(SOLAR_ERR_HIP, "hipMemset failed"));
+  // address of the no-allocation entry point's device scratch on this device
+  // (the current one), looked up once per device
+  c->scratch_status = solar::scratch_address(c->device, &c->scratch);
   c->use_graph = !env_flag("SOLAR_NO_GRAPH");
   c->profile = env_flag("SOLAR_PROFILE");
   c->shape.stage_chunks = env_int("SOLAR_STAGE_CHUNKS");
@@ -462,6 +468,31 @@ int solar_filter_exp(solar_ctx* c, const double* q, double* qf) {
   HIPCHK(solar::launch_gather(c->dt, c->q[1], c->f32, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream));
   HIPCHK(solar::launch_filter(c->dt, c->q[1], c->qw, c->f32, c->fgx, c->fgy, c->stream, c->shape));
   return download_state(c, qf, c->qw, nq);
+}
+
+int solar_compute_rhs_noalloc(solar_ctx* c, const void* q, void* q_rhs, int flags, void* stream) {
+  if (!c || !q || !q_rhs) return fail(SOLAR_ERR_ARG, "solar_compute_rhs_noalloc: null argument");
+  if (c->scratch_status != hipSuccess || !c->scratch)
+    return fail(SOLAR_ERR_HIP, std::string("solar_compute_rhs_noalloc: device scratch unavailable: ") +
+                                   hipGetErrorString(c->scratch_status));
+  // four ghost sets (p' and velocity, DDX and DDY), each 6 panels x 2 sides x 3 layers x n x nr
+  const size_t ng = static_cast<size_t>(12) * c->nr * c->n * 3;
+  const size_t set = (ng * sizeof(double) + 255) / 256 * 256;
+  if (4 * set > solar::scratch_bytes())
+    return fail(SOLAR_ERR_ARG, "solar_compute_rhs_noalloc: n*nr = " + std::to_string(c->n * c->nr) +
+                                   " needs " + std::to_string(4 * set) + " bytes of scratch, above the " +
+                                   std::to_string(solar::scratch_bytes()) + " available");
+  char* base = static_cast<char*>(c->scratch);
+  double* gxp = reinterpret_cast<double*>(base);
+  double* gyp = reinterpret_cast<double*>(base + set);
+  double* gxv = reinterpret_cast<double*>(base + 2 * set);
+  double* gyv = reinterpret_cast<double*>(base + 3 * set);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  HIPCHK(solar::launch_gather(c->dt, q, c->f32, true, 4, 1, gxp, gyp, gxv, gyv, st));
+  const int mode = 4 | (flags & (SOLAR_RHS_BC | SOLAR_RHS_SPONGE));
+  HIPCHK(solar::launch_stage(c->dt, q, q_rhs, q_rhs, c->f32, gxp, gyp, gxv, gyv, -1, 0.0, mode, st, c->shape,
+                             c->c2f));
+  return SOLAR_OK;
 }
 
 static int ghosts_common(solar_ctx* c, const double* f0, const double* f1, double* gx, double* gy, bool vec) {

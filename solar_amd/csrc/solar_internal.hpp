@@ -18,4 +18,10 @@ void clear_error();
 // *allocs and its size added to *bytes.
 int upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allocs, size_t* bytes);
 
+// The static device scratch of solar_compute_rhs_noalloc (solar_noalloc.hip):
+// its size, and its address on `device` (the current device), looked up once
+// per device.
+size_t scratch_bytes();
+hipError_t scratch_address(int device, void** addr);
+
 }  // namespace solar

@@ -36,6 +36,8 @@ EXPORTS = [
     ("solar_rk4_async", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
     ("solar_sync", ctypes.c_int, [ctypes.c_void_p]),
     ("solar_compute_rhs", ctypes.c_int, [ctypes.c_void_p, _D, _D, ctypes.c_int]),
+    ("solar_compute_rhs_noalloc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_void_p]),
     ("solar_filter_exp", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
     ("solar_scalar_ghosts", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D]),
     ("solar_vector_ghosts", ctypes.c_int, [ctypes.c_void_p, _D, _D, _D, _D]),
@@ -286,6 +288,14 @@ class Solver:
         out = np.empty(self.shape)
         _check(self.lib.solar_compute_rhs(self.h, _p(self._arr(q)), _p(out), int(flags)), "solar_compute_rhs")
         return out
+
+    def compute_rhs_noalloc(self, q_ptr, rhs_ptr, flags=0, stream=None):
+        """solar_compute_rhs_noalloc on device pointers (ints), enqueued on
+        `stream` (a hipStream_t as int, None = the handle's stream) without
+        waiting; allocates no device memory."""
+        _check(self.lib.solar_compute_rhs_noalloc(self.h, ctypes.c_void_p(q_ptr), ctypes.c_void_p(rhs_ptr),
+                                                  int(flags), ctypes.c_void_p(stream) if stream else None),
+               "solar_compute_rhs_noalloc")
 
     def filter_exp(self, q):
         out = np.empty(self.shape)
