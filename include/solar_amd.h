@@ -161,6 +161,17 @@ int solar_set_c2_field(solar_ctx* ctx, const double* c2);
  * FP32 rounding of div(v), about 1e-7 relative). */
 int solar_set_adjoint_options(solar_ctx* ctx, int div_bits);
 
+/* Divergence cache of solar_misfit_gradient with a given checkpoint spacing:
+ * enable != 0 (the default) lets the forward pass also keep the stage
+ * divergences of as many trailing segments as free device memory holds, so
+ * that only the earlier segments are recomputed from their checkpoints; 0
+ * keeps the last segment only (every earlier segment recomputed).  The
+ * gradient is the same bits either way. */
+int solar_set_adjoint_cache(solar_ctx* ctx, int enable);
+
+/* Steps the last solar_misfit_gradient recomputed from checkpoints. */
+int solar_adjoint_recomputed_steps(solar_ctx* ctx, int* steps);
+
 /* Receivers: nrec points of the p' field given as (panel, k, j, i) 0-based
  * quadruples in pkji[4*nrec]. */
 int solar_set_receivers(solar_ctx* ctx, int nrec, const int* pkji);
@@ -193,9 +204,10 @@ int solar_tangent_linear(solar_ctx* ctx, const double* q0, int nsteps, double dt
  * are cut into segments of checkpoint_every steps (<= 0: the longest that
  * fits in free device memory), the last segment ending at nsteps.  The
  * forward pass keeps the state at the start of every other segment and the
- * stage divergences div(v) of the last one; the reverse pass recomputes each
- * earlier segment once from its checkpoint, storing its divergences, and runs
- * the adjoint steps over it.  data[nsteps][nrec]; seis (may be NULL) receives
+ * stage divergences div(v) of the last one (of the last ones that fit, with
+ * the divergence cache); the reverse pass recomputes each earlier segment
+ * once from its checkpoint, storing its divergences, and runs the adjoint
+ * steps over it.  data[nsteps][nrec]; seis (may be NULL) receives
  * the modelled traces; *segment_used (may be NULL) the segment length.
  * Overwrites the handle's state.  Device memory beyond the handle's:
  * segment * 40 B per point for the divergences plus one state per earlier

@@ -75,9 +75,11 @@ struct solar_ctx {
   double *wb[2] = {nullptr, nullptr}, *tmp = nullptr, *lam = nullptr, *gc2 = nullptr;
   long long* rec = nullptr;  // receiver indices into the state (field p')
   int div_bits = 64;         // storage of the stage divergences kept for the gradient
+  bool div_cache = true;     // keep trailing segments' divergences from the forward pass in free memory
   void* adj_ws = nullptr;    // workspace of solar_misfit_gradient (checkpoints, divergences, traces), grow-only
   size_t adj_ws_bytes = 0;
   double adj_times[4] = {0, 0, 0, 0};  // last gradient: workspace, forward, recompute, adjoint (s)
+  int adj_recomputed = 0;              // last gradient: steps recomputed from checkpoints
   int nrec = 0, rec_cap = 0;
   // static device scratch of solar_compute_rhs_noalloc on this handle's device
   void* scratch = nullptr;
@@ -707,6 +709,18 @@ int solar_set_adjoint_options(solar_ctx* c, int div_bits) {
   return SOLAR_OK;
 }
 
+int solar_set_adjoint_cache(solar_ctx* c, int enable) {
+  if (!c) return fail(SOLAR_ERR_ARG, "solar_set_adjoint_cache: null handle");
+  c->div_cache = enable != 0;
+  return SOLAR_OK;
+}
+
+int solar_adjoint_recomputed_steps(solar_ctx* c, int* steps) {
+  if (!c || !steps) return fail(SOLAR_ERR_ARG, "solar_adjoint_recomputed_steps: null argument");
+  *steps = c->adj_recomputed;
+  return SOLAR_OK;
+}
+
 int solar_set_c2_field(solar_ctx* c, const double* c2) {
   if (!c) return fail(SOLAR_ERR_ARG, "solar_set_c2_field: null handle");
   if (!c2) {
@@ -879,7 +893,7 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
   const size_t margin = size_t(2) << 30;
   const size_t avail = freeb > margin ? freeb - margin : 0;
   int L = checkpoint_every > 0 ? std::min(checkpoint_every, nsteps) : nsteps;
-  if (checkpoint_every <= 0) {
+  if (checkpoint_every <= 0) {  // the longest segment that fits (no cache needed beyond it)
     while (L > 1 && need(L) > avail) --L;
     // keep the handle's workspace when it holds nearly as long a segment (a
     // new allocation of hundreds of GB costs seconds)
@@ -899,8 +913,21 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
     n1[g] = nsteps - (nseg - 1 - g) * L;
     n0[g] = std::max(0, n1[g] - L);
   }
+  // Divergence cache: with a given checkpoint spacing, free memory beyond
+  // the one-segment buffer keeps the divergences of the last C + 1 segments
+  // from the forward pass, so that only the earlier segments are recomputed.
+  int C = 0;
+  if (checkpoint_every > 0 && c->div_cache) {
+    auto need_c = [&](int cc) {
+      return need(L) - up(static_cast<size_t>(L) * step_div) + up(static_cast<size_t>(L) * (cc + 1) * step_div);
+    };
+    while (C + 1 < nseg && need_c(C + 1) <= avail) ++C;
+  }
+  const int gkeep = nseg - 1 - C;      // segments g >= gkeep keep their divergences from the forward pass
+  const int nkeep = n0[gkeep];         // first step whose divergences the forward pass stores
+  const int D = nsteps - nkeep;        // steps of divergences held (at least L)
   // one workspace, kept by the handle for the next call: checkpoints | divergences | traces | data
-  const size_t b_ck = up((nseg - 1) * ck_bytes), b_div = up(static_cast<size_t>(L) * step_div);
+  const size_t b_ck = up((nseg - 1) * ck_bytes), b_div = up(static_cast<size_t>(std::max(D, L)) * step_div);
   const size_t b_tr = up(trace_bytes / 2), ws_need = b_ck + b_div + 2 * b_tr;
   const auto t_start = std::chrono::steady_clock::now();
   if (ws_need > c->adj_ws_bytes) {
@@ -922,7 +949,7 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   for (auto& x : ev) HIPCHK(hipEventCreate(&x));
   float ms_rec = 0.f, ms_adj = 0.f, ms_fwd = 0.f;
-  std::vector<double*> dp(static_cast<size_t>(L) * 5);
+  std::vector<double*> dp(static_cast<size_t>(std::max(D, L)) * 5);
   for (size_t x = 0; x < dp.size(); ++x)
     dp[x] = reinterpret_cast<double*>(reinterpret_cast<char*>(divs) + x * nf6 * esz_div);
   const size_t nrec = static_cast<size_t>(c->nrec);
@@ -930,8 +957,9 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
     int r;
     if ((r = upload_f64(c, dat, data, static_cast<size_t>(nsteps) * nrec))) return r;
     if ((r = upload_state(c, c->q[0], q0, nq))) return r;
-    // forward pass: checkpoints, traces, and the stage divergences of the last segment
-    const int nlast = n0[nseg - 1];
+    // forward pass: checkpoints, traces, and the stage divergences of the
+    // last segment (of the last C + 1 with the cache)
+    const int nlast = nkeep;
     for (int n = 0, g = 0; n < nsteps; ++n) {
       if (g < nseg - 1 && n == n0[g]) {
         HIPCHK(hipMemcpyAsync(ck + static_cast<size_t>(g) * nq, c->q[0], ck_bytes, hipMemcpyDeviceToDevice,
@@ -961,16 +989,19 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
     HIPCHK(hipMemsetAsync(c->gc2, 0, nf6 * sizeof(double), c->stream));
     for (int g = nseg - 1; g >= 0; --g) {
       HIPCHK(hipEventRecord(ev[0], c->stream));
-      if (g < nseg - 1) {  // recompute the segment from its checkpoint, keeping the divergences
+      // divergences of step n at dp[(n - base) * 5]: cached from the forward
+      // pass, or recomputed from the segment's checkpoint into the buffer's start
+      const int base = g >= gkeep ? nkeep : n0[g];
+      if (g < gkeep) {  // recompute the segment from its checkpoint, keeping the divergences
         HIPCHK(hipMemcpyAsync(c->q[0], ck + static_cast<size_t>(g) * nq, ck_bytes, hipMemcpyDeviceToDevice,
                               c->stream));
         for (int m = n0[g]; m < n1[g]; ++m)
-          if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - n0[g]) * 5]))) return r;
+          if ((r = enqueue_step(c, dt, false, &dp[static_cast<size_t>(m - base) * 5]))) return r;
       }
       HIPCHK(hipEventRecord(ev[1], c->stream));
       for (int n = n1[g] - 1; n >= n0[g]; --n) {
         HIPCHK(solar::launch_inject(c->lam, c->rec, c->nrec, seis + n * nrec, dat + n * nrec, c->stream));
-        if ((r = enqueue_adj_step(c, dt, c->lam, &dp[static_cast<size_t>(n - n0[g]) * 5]))) return r;
+        if ((r = enqueue_adj_step(c, dt, c->lam, &dp[static_cast<size_t>(n - base) * 5]))) return r;
       }
       HIPCHK(hipEventRecord(ev[2], c->stream));
       HIPCHK(hipEventSynchronize(ev[2]));
@@ -989,6 +1020,7 @@ int solar_misfit_gradient(solar_ctx* c, const double* q0, int nsteps, double dt,
   c->adj_times[1] = ms_fwd * 1e-3;
   c->adj_times[2] = ms_rec * 1e-3;
   c->adj_times[3] = ms_adj * 1e-3;
+  c->adj_recomputed = nkeep;  // steps recomputed from checkpoints
   if (segment_used) *segment_used = L;
   return rc;
 }

@@ -47,6 +47,8 @@ EXPORTS = [
     ("solar_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
     ("solar_set_c2_field", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_set_adjoint_options", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("solar_set_adjoint_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("solar_adjoint_recomputed_steps", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     ("solar_set_receivers", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     ("solar_record", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, _D]),
     ("solar_adj_rhs", ctypes.c_int, [ctypes.c_void_p, _D, _D]),
@@ -380,6 +382,15 @@ class Solver:
     def set_adjoint_options(self, div_bits=64):
         """Storage of the stage divergences kept by misfit_gradient (64 exact, 32 half the memory)."""
         _check(self.lib.solar_set_adjoint_options(self.h, int(div_bits)), "solar_set_adjoint_options")
+
+    def set_adjoint_cache(self, enable=True):
+        _check(self.lib.solar_set_adjoint_cache(self.h, int(bool(enable))), "solar_set_adjoint_cache")
+
+    @property
+    def recomputed_steps(self):
+        v = ctypes.c_int()
+        _check(self.lib.solar_adjoint_recomputed_steps(self.h, ctypes.byref(v)), "solar_adjoint_recomputed_steps")
+        return v.value
 
     def set_receivers(self, pkji):
         """Receivers of p' at (panel, k, j, i), 0-based; array of shape (nrec, 4)."""

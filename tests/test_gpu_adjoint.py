@@ -182,6 +182,14 @@ def test_misfit_gradient_matches_finite_differences():
         # checkpoints restart the forward from the exact state, so any segmentation gives the same bits
         _, grad_one_segment, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=1000)
         assert np.array_equal(grad, grad_one_segment)
+        # the divergence cache holds every segment here (nothing recomputed); without it the four
+        # earlier segments of 5 steps are recomputed from their checkpoints: the same bits
+        assert s.recomputed_steps == 0
+        s.set_adjoint_cache(False)
+        _, grad_recomputed, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=5)
+        assert s.recomputed_steps == steps - 5
+        s.set_adjoint_cache(True)
+        assert np.array_equal(grad, grad_recomputed)
         # FP32 storage of the stage divergences: the gradient carries their rounding only
         s.set_adjoint_options(32)
         _, grad32, _ = s.misfit_gradient(q0, steps, dt, data, checkpoint_every=5)

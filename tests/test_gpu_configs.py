@@ -206,9 +206,15 @@ def test_c5_tangent_linear_dot_product(c5):
 
 
 def test_c5_gradient_independent_of_checkpoint_spacing(c5):
+    """The same bits for any checkpoint spacing, and with the divergence cache
+    off (every earlier segment recomputed from its checkpoint)."""
     s, p = c5
     s.set_c2_field(p["c2_0"])
-    for every in (0, 37):
+    for every, cache in ((0, True), (37, True), (50, False)):
+        s.set_adjoint_cache(cache)
         _, g, _ = s.misfit_gradient(p["q0"], C5_STEPS, p["dt"], p["data"], every)
-        print(f"[c5] checkpoint_every={every}: segment {s.segment_used}", flush=True)
+        print(f"[c5] checkpoint_every={every} cache={cache}: segment {s.segment_used}, "
+              f"recomputed {s.recomputed_steps} steps", flush=True)
         assert np.array_equal(g, p["grad"]), every
+    assert s.recomputed_steps == C5_STEPS - 50
+    s.set_adjoint_cache(True)
