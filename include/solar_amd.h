@@ -50,7 +50,10 @@ enum {
  * (ic_acoustic_wave.f90:94-108, radial-only there, so passed as profiles).
  *   n          angular points per panel edge (nx = ny), 16 <= n <= 512 (any n;
  *              the sensitivity entry points and ensembles need a multiple of 16)
- *   nr         radial points, >= 12
+ *   nr         radial points, >= 12, or 1: the reference's 2-D case (no radial
+ *              derivative, FILTER_R, BC_RHS or SPONGE, and r = 1: derivative.f90:620-623,
+ *              filter.f90:647-650, bc_acoustic_wave.f90:11, sponge.f90:18,
+ *              metric.f90:35-38; forward only)
  *   gr, c2, rho0, g0   radial profiles, length nr (nondimensional)
  *   sponge_len, a_sponge   sponge.f90:27,20 (k_sponge_length, a_sponge)
  */

@@ -93,12 +93,21 @@ static void setup_metrics(orc_t* o) {
     o->geta[i - 1] = -pi * 0.25 + (((double)i - 1.0) * pi * 0.5) / ((double)N - 1.0);
   }
   for (int i = 1; i <= R; i++) o->gchi[i - 1] = 0.0 + (((double)i - 1.0) * 1.0) / ((double)R - 1.0);
+  if (R == 1) {
+    /* 2-D: metric.f90:20,45 would read 0/0 and chi(2), which does not exist;
+     * nothing radial is used (DDR returns 0, derivative.f90:620-623), so the
+     * radial metric is left at 0. */
+    o->gchi[0] = 0.0;
+    o->dchi = 0.0;
+    o->drdchi[0] = 0.0;
+  }
   for (int i = 0; i < N; i++) {
     o->tanxi[i] = tan(o->gxi[i]);
     o->taneta[i] = tan(o->geta[i]);
   }
   o->dxi = o->gxi[1] - o->gxi[0];
   o->deta = o->dxi;
+  if (R == 1) goto angular;
   o->dchi = o->gchi[1] - o->gchi[0];
 
   /* metric.f90:48-67 gdrdchi */
@@ -119,6 +128,7 @@ static void setup_metrics(orc_t* o) {
     o->drdchi[kk - 1] = s;
   }
 
+angular:
   /* metric.f90:74-85 */
   for (int i = 0; i < N; i++) {
     o->mC[i] = pow(1.0 + o->tanxi[i] * o->tanxi[i], 0.5);
@@ -181,7 +191,7 @@ static void setup_interpolant(orc_t* o) {
 
 orc_t* orc_create(int N, int R, const double* gr, const double* c2, const double* rho0, const double* g0,
                   int sponge_len, double a_sponge) {
-  if (N < 9 || R < 12) return NULL;
+  if (N < 9 || (R < 12 && R != 1)) return NULL;  /* R == 1: the reference's 2-D case */
   orc_t* o = (orc_t*)calloc(1, sizeof(orc_t));
   o->N = N;
   o->R = R;
@@ -197,6 +207,7 @@ orc_t* orc_create(int N, int R, const double* gr, const double* c2, const double
   o->interpoints = (double*)calloc((size_t)N * 3, sizeof(double));
   o->interpolant = (double*)calloc((size_t)N * 27, sizeof(double));
   memcpy(o->gr, gr, sizeof(double) * R);
+  if (R == 1) o->gr[0] = 1.0; /* metric.f90:35-38: the r effect is nullified in 2-D */
   memcpy(o->c2, c2, sizeof(double) * R);
   memcpy(o->rho0, rho0, sizeof(double) * R);
   memcpy(o->g0, g0, sizeof(double) * R);
@@ -468,6 +479,10 @@ static void ddy(const orc_t* o, const double* f, double* d, const double* fp, co
  * both shell ends.  f, d: (nx,ny,R) with nx*ny = ncol columns. */
 static void ddr_cols(const orc_t* o, const double* f, double* d, size_t ncol) {
   const int R = o->R;
+  if (R == 1) { /* 2-D case, derivative.f90:620-623: dr_f = 0 */
+    for (size_t c = 0; c < ncol; c++) d[c] = 0.0;
+    return;
+  }
   const double fac = 1.0 / o->dchi;
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= R; k++) {
@@ -825,6 +840,7 @@ void orc_compute_rhs(const orc_t* o, const double* q, double* rhs) {
 /* bc_acoustic_wave.f90:14-34 (rrank=0=pr-1). */
 void orc_bc_rhs(const orc_t* o, double* rhs) {
   const int N = o->N, R = o->R;
+  if (R == 1) return; /* bc_acoustic_wave.f90:11 */
   const size_t nf = (size_t)N * N * R;
   for (int z = 0; z < 6; z++)
     for (int l = 0; l < 5; l++) {
@@ -837,6 +853,7 @@ void orc_bc_rhs(const orc_t* o, double* rhs) {
 /* sponge.f90:20-50: single-precision ratio, applied to fields 1, 4, 5. */
 double orc_sponge_sigma(const orc_t* o, int gk /* 1-based */) {
   const int L = o->sponge_len;
+  if (o->R == 1) return 0.0; /* sponge.f90:18: no sponge in 2-D */
   const int lower = L, upper = o->R - L + 1;
   volatile float num, den;
   if (gk >= upper) {
@@ -855,6 +872,7 @@ double orc_sponge_sigma(const orc_t* o, int gk /* 1-based */) {
 
 void orc_sponge(const orc_t* o, const double* q, double* rhs) {
   const int N = o->N, R = o->R;
+  if (R == 1) return; /* sponge.f90:18 */
   const size_t nf = (size_t)N * N * R;
   for (int k = 0; k < R; k++) {
     const double sg = orc_sponge_sigma(o, k + 1);
@@ -937,6 +955,10 @@ static void filter_eta(const orc_t* o, const double* g, double* gf, const double
 static void filter_r(const orc_t* o, const double* g, double* gf) {
   const int N = o->N, R = o->R;
   const size_t plane = (size_t)N * N;
+  if (R == 1) { /* 2-D case, filter.f90:647-650: gf = g */
+    memcpy(gf, g, sizeof(double) * plane);
+    return;
+  }
   memset(gf, 0, sizeof(double) * plane * R);
 #pragma omp parallel for schedule(static)
   for (int k = NSTR + 1; k <= R - NSTR; k++)

@@ -649,6 +649,7 @@ static int adj_alloc(solar_ctx* c) {
   if (c->n % 16 != 0)
     return fail(SOLAR_ERR_ARG, "adjoint: needs a panel size n that is a multiple of 16 (n = " + std::to_string(c->n) +
                                    "); the forward path takes any n");
+  if (c->nr < 12) return fail(SOLAR_ERR_ARG, "adjoint: needs nr >= 12 (the 2-D case nr = 1 runs forward only)");
   if (c->lam) return SOLAR_OK;
   if (c->f32) return fail(SOLAR_ERR_ARG, "adjoint: needs a 64-bit state handle");
   const size_t nq = c->nf * 30;

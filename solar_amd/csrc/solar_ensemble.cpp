@@ -249,6 +249,9 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
     if (n[m] % 16 != 0)  // the ensemble's work lists tile every member exactly
       return bail(solar::set_error(SOLAR_ERR_ARG, "solar_ens_create: member " + std::to_string(m) +
                                                       ": n must be a multiple of 16 in an ensemble"));
+    if (nr[m] < 12)
+      return bail(solar::set_error(SOLAR_ERR_ARG, "solar_ens_create: member " + std::to_string(m) +
+                                                      ": nr must be at least 12 in an ensemble"));
     if (!solar::build_tables(n[m], nr[m], gr + roff, c2 + roff, rho0 + roff, g0 + roff, sponge_len[m], a_sponge[m],
                              &t, &err))
       return bail(solar::set_error(SOLAR_ERR_ARG, "solar_ens_create: member " + std::to_string(m) + ": " + err));

@@ -475,9 +475,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   S pw[7], rw[7];
   int kb = clampi(k0 - 3, 0, nr - 7);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    pw[q] = f4[(size_t)(kb + q) * plane + ij];
-    rw[q] = f3[(size_t)(kb + q) * plane + ij];
+  for (int q = 0; q < 7; ++q) {  // (nr == 1, the 2-D case: every slot holds plane 0, with DDR weights 0)
+    const int kq = min(kb + q, nr - 1);
+    pw[q] = f4[(size_t)kq * plane + ij];
+    rw[q] = f3[(size_t)kq * plane + ij];
   }
   // The next plane is loaded while this one is computed (a second plane in
   // flight was measured: no gain for FP64 or FP32 storage).  Each iteration
@@ -609,8 +610,9 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     if constexpr (TL) r5 = r5 - (cur.tdc2 * row.rho0) * cur.tdiv;
     r5 = r5 + row.rho0g0 * rc;
 
-    const bool do_bc = rhs_mode ? (rhs_mode & 1) : true;
-    const bool do_sp = rhs_mode ? (rhs_mode & 2) : true;
+    // no BC_RHS and no SPONGE in 2-D (bc_acoustic_wave.f90:11, sponge.f90:18)
+    const bool do_bc = (rhs_mode ? (rhs_mode & 1) : true) && nr > 1;
+    const bool do_sp = (rhs_mode ? (rhs_mode & 2) : true) && nr > 1;
     if (do_bc && (k == 0 || k == nr - 1)) {  // bc_acoustic_wave.f90:14-34
       r1 = 0.0; r2 = 0.0; r3 = 0.0; r4 = 0.0; r5 = 0.0;
     }
@@ -693,11 +695,13 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   const double* gyS = fgy + F * fstride + (size_t)(P * 2) * gy_side;
   const size_t ij = own_in ? (size_t)j * n + i : 0;
 
+  // 2-D case (nr == 1): FILTER_R returns its input (filter.f90:647-650)
+  const bool twod = nr == 1;
   // filter.f90:652,726-777: the five planes at each end are zero.
-  if (own_in)
+  if (own_in && !twod)
     for (int k = k0; k < k1; ++k)
       if (k < kNstr || k >= nr - kNstr) o[(size_t)k * plane + ij] = static_cast<S>(0.0);
-  const int a = max(k0, kNstr), bnd = min(k1, nr - kNstr);
+  const int a = twod ? 0 : max(k0, kNstr), bnd = twod ? 1 : min(k1, nr - kNstr);
   if (a >= bnd) return;
 
   // Tile cells this thread fills: kind 1 a field cell (fp, plane stride
@@ -786,6 +790,35 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     const double v = kind[c] == 1 ? fv : L.g[c];
     return kind[c] ? v : 0.0;
   };
+
+  if (twod) {  // the one plane: FILTER_XI, FILTER_ETA, stored as they come
+    FPlane p0;
+    load_plane(0, p0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = tid + c * NT;
+      if (e < NCELL) (&sq[0][0])[e] = widen(p0, c);
+    }
+    __syncthreads();
+    for (int rr = ty; rr < TJ + 6; rr += TJ) {
+      const int gj = j0 - 3 + rr;
+      double xv;
+      if (gj >= 0 && gj < n) {
+        xv = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
+      } else {
+        xv = sq[rr][tx + 3];
+      }
+      sx[rr][tx] = xv;
+    }
+    __syncthreads();
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) v = v + T.filt[q] * sx[ty + q][tx];
+    if (own_in) o[ij] = static_cast<S>(v);
+    return;
+  }
 
   double win[11];
 #pragma unroll

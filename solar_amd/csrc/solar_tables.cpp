@@ -67,11 +67,12 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
     if (err) *err = "panel size n must lie in [16, 512] (got " + std::to_string(n) + ")";
     return false;
   }
-  if (nr < 12) {
-    if (err) *err = "radial size nr must be at least 12 (got " + std::to_string(nr) + ")";
+  if (nr < 12 && nr != 1) {  // nr == 1: the reference's 2-D case
+    if (err) *err = "radial size nr must be 1 or at least 12 (got " + std::to_string(nr) + ")";
     return false;
   }
-  if (sponge_len < 1 || 2 * sponge_len > nr) {
+  const bool twod = nr == 1;
+  if (!twod && (sponge_len < 1 || 2 * sponge_len > nr)) {
     if (err) *err = "sponge length must lie in [1, nr/2]";
     return false;
   }
@@ -95,6 +96,13 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
   for (int i = 1; i <= nr; ++i)
     t->gchi[i - 1] = 0.0 + ((static_cast<double>(i) - 1.0) * 1.0) / (static_cast<double>(nr) - 1.0);
   t->gr.assign(gr, gr + nr);
+  if (twod) {
+    // metric.f90:35-38 nullifies the r effect in 2-D (r = 1); the radial
+    // metric (0/0 and chi(2) there) is unused, as DDR returns 0
+    // (derivative.f90:620-623), and is left at 0.
+    t->gr[0] = 1.0;
+    t->gchi[0] = 0.0;
+  }
   t->tanxi.resize(n);
   t->taneta.resize(n);
   for (int i = 0; i < n; ++i) {
@@ -103,12 +111,12 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
   }
   t->dxi = t->gxi[1] - t->gxi[0];
   t->deta = t->dxi;
-  t->dchi = t->gchi[1] - t->gchi[0];
+  t->dchi = twod ? 0.0 : t->gchi[1] - t->gchi[0];
 
   // drdchi: the DRP derivative of r(chi), metric.f90:48-67.
   t->drdchi.assign(nr, 0.0);
-  const double inv_dchi = 1.0 / t->dchi;
-  for (int k = 0; k < nr; ++k) {
+  const double inv_dchi = twod ? 0.0 : 1.0 / t->dchi;
+  for (int k = 0; k < nr && !twod; ++k) {
     double s = 0.0;
     if (k >= kNst && k < nr - kNst) {
       for (int q = 0; q < 7; ++q) s = s + (inv_dchi * kCoeffs1[q]) * gr[k - kNst + q];
@@ -282,8 +290,8 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
 
   // Radial rows: background, DDR weights, sponge.
   t->radial.assign(nr, RadialRow{});
-  std::vector<double> wk(static_cast<size_t>(nr) * 7);
-  for (int k = 0; k < nr; ++k) {
+  std::vector<double> wk(static_cast<size_t>(nr) * 7, 0.0);  // 2-D: DDR weights 0
+  for (int k = 0; k < nr && !twod; ++k) {
     double* w = &wk[static_cast<size_t>(k) * 7];
     if (k >= kNst && k < nr - kNst) {
       const double fac2 = inv_dchi / t->drdchi[k];
@@ -298,17 +306,19 @@ bool build_tables(int n, int nr, const double* gr, const double* c2, const doubl
   }
   for (int k = 0; k < nr; ++k) {
     RadialRow& row = t->radial[k];
-    row.inv_r = 1.0 / gr[k];
-    row.r = gr[k];
+    row.inv_r = 1.0 / t->gr[k];
+    row.r = t->gr[k];
     row.rho0 = rho0[k];
     row.c2rho0 = c2[k] * rho0[k];
     row.g0 = g0[k];
     row.rho0g0 = rho0[k] * g0[k];
-    const int base = (k < kNst) ? 0 : (k >= nr - kNst ? nr - 7 : k - kNst);
-    double s = 0.0;
-    for (int q = 0; q < 7; ++q) s = s + wk[static_cast<size_t>(k) * 7 + q] * rho0[base + q];
+    double s = 0.0;  // DDR(rho0), compute_rhs_acoustic.f90:51; 0 in 2-D
+    if (!twod) {
+      const int base = (k < kNst) ? 0 : (k >= nr - kNst ? nr - 7 : k - kNst);
+      for (int q = 0; q < 7; ++q) s = s + wk[static_cast<size_t>(k) * 7 + q] * rho0[base + q];
+    }
     row.drho0dr = s;
-    row.sigma = sponge_sigma(k + 1, nr, sponge_len, a_sponge);
+    row.sigma = twod ? 0.0 : sponge_sigma(k + 1, nr, sponge_len, a_sponge);  // sponge.f90:18
     for (int q = 0; q < 7; ++q) row.w[q] = wk[static_cast<size_t>(k) * 7 + q];
     row.inv_rho0 = 1.0 / row.rho0;
   }

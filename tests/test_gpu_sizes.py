@@ -82,3 +82,26 @@ def test_adjoint_and_ensemble_refuse_untiled_sizes():
     member = dict(n=100, nr=14, gr=gr, c2=c2, rho0=rho0, g0=g0, sponge_len=L, a_sponge=0.1, dt=dt)
     with pytest.raises(SolarError, match="multiple of 16"):
         Ensemble([member])
+
+
+@pytest.mark.parametrize("n", [32, 100])
+def test_two_dimensional_case_bit_exact(n):
+    """nr = 1, the reference's 2-D case: no radial derivative, FILTER_R, BC_RHS
+    or SPONGE, and r = 1.  RHS, FILTER_EXP and RK4 steps against the CPU
+    restatement, bit for bit (n = 100 also runs partial tiles)."""
+    gr, c2, rho0, g0 = (np.array([v]) for v in (0.85, 1.3, 0.9, 0.4))
+    rng = np.random.default_rng(n)
+    q = rng.standard_normal((6, 5, 1, n, n))
+    dt = 0.5 * (np.pi / 2) / (n - 1) / np.sqrt(1.3)
+    print(f"[case] 2-D n={n} nr=1", flush=True)
+    o = oracle.Oracle(n, 1, gr, c2, rho0, g0, 1, 0.1)
+    with Solver(n, 1, gr, c2, rho0, g0, 1, 0.1) as s:
+        _same(s.compute_rhs(q, RHS_BC | RHS_SPONGE), o.stage_rhs(q), f"2-D n={n} rhs")
+        _same(s.filter_exp(q), o.filter_exp(q), f"2-D n={n} filter")
+        s.set_state(q)
+        s.rk4(5, dt)
+        got = s.get_state()
+        with pytest.raises(SolarError, match="multiple of 16|nr >= 12"):
+            s.adj_rhs(q)
+    want, _ = o.rk4(q, 5, dt)
+    _same(got, want, f"2-D n={n} rk4 x5")

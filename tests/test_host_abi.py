@@ -27,9 +27,11 @@ def test_library_exports_every_declared_symbol():
     assert names == {e[0] for e in EXPORTS}
 
 
-@pytest.mark.parametrize("n,nr,kind", [(16, 24, 1), (32, 64, 0), (48, 20, 1), (96, 40, 1), (20, 24, 1), (100, 30, 1)])
+@pytest.mark.parametrize("n,nr,kind", [(16, 24, 1), (32, 64, 0), (48, 20, 1), (96, 40, 1), (20, 24, 1), (100, 30, 1),
+                                       (24, 1, 1)])
 def test_host_tables_equal_oracle_tables(n, nr, kind):
-    gr, c2, rho0, g0 = background(kind, nr)
+    # nr = 1: the 2-D case, one shell (its radius is replaced by 1)
+    gr, c2, rho0, g0 = background(kind, nr) if nr > 1 else (np.array([v]) for v in (0.85, 1.3, 0.9, 0.4))
     L = max(1, round(0.08 * nr))
     o = oracle.Oracle(n, nr, gr, c2, rho0, g0, L, 0.1)
     t = o.tables()
@@ -47,8 +49,9 @@ def test_invalid_sizes_rejected():
     for n in (15, 513):
         with pytest.raises(solar_amd.SolarError, match=r"\[16, 512\]"):
             host_tables(n, 24, gr, c2, rho0, g0, 2, 0.1)
-    with pytest.raises(solar_amd.SolarError, match="at least 12"):
-        host_tables(16, 11, gr[:11], c2[:11], rho0[:11], g0[:11], 2, 0.1)
+    for nr in (2, 11):
+        with pytest.raises(solar_amd.SolarError, match="1 or at least 12"):
+            host_tables(16, nr, gr[:nr], c2[:nr], rho0[:nr], g0[:nr], 1, 0.1)
     with pytest.raises(solar_amd.SolarError, match="sponge"):
         host_tables(16, 24, gr, c2, rho0, g0, 20, 0.1)
 

@@ -545,3 +545,55 @@ def test_filter_exp_treats_velocities_as_scalars():
             for other in range(5):
                 if other != fld:
                     assert not np.any(got[:, other]), (p, fld, other)
+
+
+# ------------------------------------------------------------------ 2-D case
+# nr == 1 (derivative.f90:620-623: DDR = 0; filter.f90:647-650: FILTER_R
+# returns its input; bc_acoustic_wave.f90:11, sponge.f90:18: no BC, no sponge;
+# metric.f90:35-38: r = 1 whatever the radius passed).
+
+def _oracle_2d(n=24):
+    gr, c2, rho0, g0 = (np.array([v]) for v in (0.85, 1.3, 0.9, 0.4))
+    return oracle.Oracle(n, 1, gr, c2, rho0, g0, 1), (1.3, 0.9, 0.4)
+
+
+def test_2d_rhs_closed_forms():
+    o, (c2, rho0, g0) = _oracle_2d()
+    n = o.n
+    q = np.zeros((6, 5, 1, n, n))
+    q[:, 3] = 0.7    # v_r alone: div = 2 v_r / r with r = 1, no radial derivative
+    q[:, 0] = -0.25  # rho' alone enters dv_r/dt through buoyancy only
+    rhs = o.compute_rhs(q)
+    assert np.array_equal(rhs[:, 0], np.full_like(rhs[:, 0], -(rho0 * (2.0 * 0.7)) - 0.7 * 0.0))
+    assert np.array_equal(rhs[:, 3], np.full_like(rhs[:, 3], -(0.0 / rho0) - (-0.25 * g0) / rho0))
+    assert np.array_equal(rhs[:, 4], np.full_like(rhs[:, 4], -((c2 * rho0) * (2.0 * 0.7)) + (rho0 * g0) * 0.7))
+    assert np.array_equal(rhs[:, 1:3], np.zeros_like(rhs[:, 1:3]))
+    # a radius other than 1 is not used: the 2 v_r / r term above holds with r = 1
+    # BC_RHS and SPONGE do nothing in 2-D: the stage RHS is COMPUTE_RHS itself
+    qr = np.random.default_rng(3).standard_normal(q.shape)
+    assert np.array_equal(o.stage_rhs(qr), o.compute_rhs(qr))
+    assert o.sponge_sigma(1) == 0.0
+
+
+def test_2d_filter_is_xi_then_eta_without_radial_pass():
+    """An interior impulse filters into the 7x7 outer product of the filter
+    coefficients, formed xi first (filt[b] * a) then eta (filt[a] * that), and
+    FILTER_R leaves it alone (in 3-D it would scale the centre plane by
+    0.753906 and zero the shell's 5 end planes)."""
+    o, _ = _oracle_2d()
+    n = o.n
+    filt = np.array([0.015625, -0.09375, 0.234375, 0.6875, 0.234375, -0.09375, 0.015625])
+    q = np.zeros((6, 5, 1, n, n))
+    amp, pj, pi_ = 3.0, 11, 9
+    q[2, 4, 0, pj, pi_] = amp
+    qf = o.filter_exp(q)
+    want = np.zeros((n, n))
+    for a in range(-3, 4):
+        for b in range(-3, 4):
+            want[pj + a, pi_ + b] = filt[a + 3] * (filt[b + 3] * amp)
+    assert np.array_equal(qf[2, 4, 0], want)
+    qf[2, 4, 0] = 0.0
+    assert not np.any(qf)
+    # a constant field stays constant (ghosts interpolate it to rounding); no plane is zeroed
+    qc = np.full((6, 5, 1, n, n), 2.5)
+    assert np.abs(o.filter_exp(qc) - 2.5).max() < 1e-13
