@@ -15,7 +15,9 @@ Printed as one JSON line, with
   roofline     : the dominant kernel (the fused stage kernel), algorithmic
                  bytes per launch / its mean launch time (HIP events on the
                  library's stream), against the 8 TB/s HBM peak; plus the whole
-                 step against 800 B per point per step (step_frac)
+                 step against 800 B per point per step (step_frac, the 2N
+                 scheme storing every stage) and against the bytes the step
+                 moves (step_frac_moved: 720 B in FP64)
   cpu_baseline : the CPU restatement of the reference (oracle/) on this host,
                  on a bounded sample of the same workload.
 """
@@ -30,10 +32,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grid-point RK4 steps/s (FP64, % HBM roofline) at 1 GPU; fwd+adjoint vs plain fwd cost"
 HBM_PEAK = 8.0e12
-# Algorithmic HBM bytes per grid point (FP64 state of 5 fields = 40 B):
+# The roofline basis of the metric: the 2N step storing q and qw after every
+# stage (FP64 state of 5 fields = 40 B per point):
 #   stage 1 and 5: read q + write q and qw / read q and qw + write q -> 120 B
 #   stages 2-4: read q and qw, write q and qw -> 160 B;  filter: 80 B
-STAGE_BYTES = (120, 160, 160, 160, 120)
+# (half for a 32-bit state).  The library's FP64 step moves less (it stores q
+# after stages 2, 4 and 5 only: 80, 160, 120, 160, 120 + 80 = 720 B); the
+# dominant kernel's roofline uses the bytes its launches really move, which
+# the handle reports (Solver.algorithmic_bytes).
 STEP_BYTES = 800
 
 DESCR = {
@@ -194,12 +200,14 @@ def _grid_bench(name, steps, cpu_steps, bits=64):
     p.rk4(2, dt)
     p.rk4(5, dt)
     kt = p.kernel_times()
+    ab = p.algorithmic_bytes()
     p.close()
     rate = cfg.points * steps / wall
     scale = bits / 64.0
-    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points * scale
+    stage_bytes = sum(ab[:5]) / 5.0 * cfg.points
     out = {"workload": _workload_descr(name, bits), "steps": steps, "value": rate, "unit": "grid-point RK4 steps/s",
            "ms_per_step": wall / steps * 1e3, "step_frac": rate * STEP_BYTES * scale / HBM_PEAK,
+           "step_bytes_per_point": sum(ab), "step_frac_moved": rate * sum(ab) / HBM_PEAK,
            "stage_frac": stage_bytes / (kt["stage_ms"] * 1e-3) / HBM_PEAK, "state_finite": ok,
            "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
     if cpu_steps:
@@ -315,9 +323,10 @@ def main():
     p.rk4(2, dt)
     p.rk4(args.profile_steps, dt)
     kt = p.kernel_times()
+    ab = p.algorithmic_bytes()  # per point: five stages, filter
     p.close()
-    scale = bits / 64.0  # FP32 storage halves the algorithmic bytes
-    stage_bytes = sum(STAGE_BYTES) / 5.0 * cfg.points * scale
+    scale = bits / 64.0  # FP32 storage halves the roofline basis
+    stage_bytes = sum(ab[:5]) / 5.0 * cfg.points
     stage_s = kt["stage_ms"] * 1e-3
     achieved = stage_bytes / stage_s
     traffic = None
@@ -330,7 +339,10 @@ def main():
                 "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "kernel": "k_stage (fused RK stage)", "kernel_ms": kt["stage_ms"],
                 "algorithmic_bytes_per_launch": stage_bytes,
+                "stage_bytes_per_point": list(ab[:5]),
                 "step_frac": rate * STEP_BYTES * scale / HBM_PEAK,
+                "step_bytes_per_point": sum(ab),
+                "step_frac_moved": rate * sum(ab) / HBM_PEAK,
                 "kernel_ms_all": {k: round(v, 5) for k, v in kt.items()}}
 
     precision = None

@@ -146,6 +146,16 @@ int solar_kernel_times(solar_ctx* ctx, double* out);
 /* Bytes of device memory held by the handle. */
 int solar_device_bytes(solar_ctx* ctx, size_t* bytes);
 
+/* Algorithmic HBM bytes per grid point of each launch of this handle's RK4
+ * step (state reads and writes, without halos, ghosts or tables):
+ * out[0..4] the five stages, out[5] the filter.  With a 64-bit state the
+ * step stores q after stages 2, 4 and 5 only (stages 2 and 4 rebuild their
+ * input from q and the accumulator): 80, 160, 120, 160, 120 and 80 bytes,
+ * 720 per step; with SOLAR_STEP_STORE_ALL=1 at solar_init, or a 32-bit
+ * state, every stage stores q: 120, 160, 160, 160, 120, 80 (800; half that
+ * for 32-bit).  No counterpart in the reference (for benchmarks). */
+int solar_algorithmic_bytes(solar_ctx* ctx, double* out);
+
 /* Sensitivities (discrete adjoint).  No counterpart in the reference, which
  * has no adjoint: these serve inversion and data-assimilation callers of the
  * same RK4 step.  All need a 64-bit state handle. ----------------------- */

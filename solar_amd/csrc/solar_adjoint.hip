@@ -865,6 +865,7 @@ static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, 
   int nkch = (2048 + tiles - 1) / tiles;
   const int maxk = T.nr / 24 > 0 ? T.nr / 24 : 1;
   if (nkch > maxk) nkch = maxk;
+  if (a.chunks > 0) nkch = std::min(a.chunks, T.nr);
   if (nkch < 1) nkch = 1;
   const int kchunk = (T.nr + nkch - 1) / nkch;
   hipLaunchKernelGGL((k_adj_stage<TI, TJ, D32>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, a.W, a.Yin, a.Yout,
@@ -928,12 +929,13 @@ hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, c
 
 template <int TI, int TJ>
 static hipError_t launch_adj_filter_t(const DevTables& T, const double* in, double* out, double* fgxa, double* fgya,
-                                      hipStream_t st) {
+                                      hipStream_t st, int chunks) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 30;
   // chunks of >= 48 planes: the march re-reads 10 planes per chunk
   int nkch = (2048 + tiles - 1) / tiles;
   const int maxk = T.nr / 48 > 0 ? T.nr / 48 : 1;
   if (nkch > maxk) nkch = maxk;
+  if (chunks > 0) nkch = std::min(chunks, T.nr);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   hipLaunchKernelGGL((k_adj_filter<TI, TJ>), dim3(tiles * nkch), dim3(TI, TJ), 0, st, T, in, out, kchunk, nkch);
   const size_t lds = sizeof(double) * 2 * 3 * (size_t)T.n;
@@ -946,9 +948,9 @@ static hipError_t launch_adj_filter_t(const DevTables& T, const double* in, doub
 }
 
 hipError_t launch_adj_filter(const DevTables& T, const double* in, double* out, double* fgxa, double* fgya,
-                             hipStream_t st) {
-  if (T.n % 32 == 0) return launch_adj_filter_t<32, 16>(T, in, out, fgxa, fgya, st);
-  return launch_adj_filter_t<16, 16>(T, in, out, fgxa, fgya, st);
+                             hipStream_t st, int chunks) {
+  if (T.n % 32 == 0) return launch_adj_filter_t<32, 16>(T, in, out, fgxa, fgya, st, chunks);
+  return launch_adj_filter_t<16, 16>(T, in, out, fgxa, fgya, st, chunks);
 }
 
 hipError_t launch_sample(const double* q, const long long* idx, int nrec, double* out, hipStream_t st) {

@@ -22,6 +22,7 @@ struct AdjStageArgs {
   double dt = 1.0, alpha = 0.0, beta = 0.0, wscale = 1.0;
   int mode = 0;
   int div32 = 0;  // divs holds float values
+  int chunks = 0;  // radial chunks per tile column (0: automatic)
 };
 
 hipError_t launch_adj_stage(const DevTables& T, const AdjStageArgs& a, hipStream_t st);
@@ -34,8 +35,9 @@ hipError_t launch_adj_scatter(const DevTables& T, bool vec, const double* gxs, c
                               hipStream_t st, bool radial_filter = false);
 // out = FILTER_EXP^T(in) without the ghost contributions, which go to
 // fgxa/fgya per plane (to be scattered with radial_filter = true).
+// chunks: radial chunks per tile column (0: automatic).
 hipError_t launch_adj_filter(const DevTables& T, const double* in, double* out, double* fgxa, double* fgya,
-                             hipStream_t st);
+                             hipStream_t st, int chunks = 0);
 hipError_t launch_sample(const double* q, const long long* idx, int nrec, double* out, hipStream_t st);
 hipError_t launch_inject(double* lam, const long long* idx, int nrec, const double* seis, const double* data,
                          hipStream_t st);

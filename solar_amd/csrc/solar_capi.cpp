@@ -53,6 +53,7 @@ struct solar_ctx {
   std::vector<void*> allocs;
   void* q[2] = {nullptr, nullptr};  // state ping-pong, element type per f32
   void* qw = nullptr;               // 2N accumulator, same type
+  void* qw2 = nullptr;              // second accumulator of the stored-q schedule (FP64 state)
   double *gxp = nullptr, *gyp = nullptr, *gxv = nullptr, *gyv = nullptr;
   double *fgx = nullptr, *fgy = nullptr;
   void* ecols = nullptr;  // edge-column copies written by the stages for the next gather (state type)
@@ -131,6 +132,17 @@ void release(solar_ctx* c) {
 // divp: where stage s stores div(v) of its input.  b0, b1, bw: other state
 // and accumulator buffers (a tangent-linear state); tl_dc2 with tl_divp: the
 // tangent-linear source of each stage (primal divergences tl_divp[s]).
+//
+// With an FP64 state the step stores q after stages 2, 4 and 5 only (720
+// instead of 800 bytes per point): stages 1 and 3 store their accumulator
+// and the edge columns of q, and stages 2 and 4 rebuild their input as
+// q + beta*qw wherever they read it, with the expression that produced it,
+// so every value is the same as when q is stored.  Buffers, A = q[0], B =
+// q[1], X = qw, Y = qw2 (a rebuilding stage reads its qw at other blocks'
+// halo cells, so it writes its own to the other accumulator):
+//   stage 1  A -> qw X                 stage 2  A + b1 X -> q B, qw Y
+//   stage 3  B, Y -> qw Y              stage 4  B + b3 Y -> q A, qw X
+//   stage 5  A, X -> q B               filter   B -> A
 int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = nullptr, void* b0 = nullptr,
                  void* b1 = nullptr, void* bw = nullptr, const double* tl_dc2 = nullptr,
                  double* const* tl_divp = nullptr) {
@@ -139,6 +151,42 @@ int enqueue_step(solar_ctx* c, double dt, bool timed, double* const* divp = null
   void* nxt = b1 ? b1 : c->q[1];
   void* qw = bw ? bw : c->qw;
   int e = 0;
+  if (c->qw2 && !b0 && !tl_dc2) {
+    void* A = c->q[0];
+    void* B = c->q[1];
+    void* X = c->qw;
+    void* Y = c->qw2;
+    // per stage: gathered input (q, rebuilt from qw with beta), stage buffers
+    struct St {
+      void *qg, *wg;
+      double bg;
+      void *qin, *qout, *qw, *qwo;
+      int xf;
+    };
+    const St sch[5] = {
+        {A, nullptr, 0.0, A, B, X, nullptr, solar::kStoreNoQ},  // (q out: not stored)
+        {A, X, solar::kBetaRk4[0], A, B, X, Y, solar::kStageRecon},
+        {B, nullptr, 0.0, B, A, Y, nullptr, solar::kStoreNoQ},
+        {B, Y, solar::kBetaRk4[2], B, A, Y, X, solar::kStageRecon},
+        {A, nullptr, 0.0, A, B, X, nullptr, 0},
+    };
+    for (int s = 0; s < 5; ++s) {
+      const St& x = sch[s];
+      if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+      HIPCHK(solar::launch_gather(T, x.qg, false, true, 4, 1, c->gxp, c->gyp, c->gxv, c->gyv, c->stream,
+                                  s > 0 ? c->ecols : nullptr, x.wg, x.bg));
+      if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+      HIPCHK(solar::launch_stage(T, x.qin, x.qout, x.qw, false, c->gxp, c->gyp, c->gxv, c->gyv, s, dt,
+                                 0, c->stream, c->shape, c->c2f, divp ? divp[s] : nullptr, c->div_bits == 32,
+                                 nullptr, nullptr, c->ecols, x.xf, x.qwo, x.bg));
+    }
+    if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+    HIPCHK(solar::launch_gather(T, B, false, false, 0, 5, c->fgx, c->fgy, nullptr, nullptr, c->stream, c->ecols));
+    if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+    HIPCHK(solar::launch_filter(T, B, A, false, c->fgx, c->fgy, c->stream, c->shape));
+    if (timed) HIPCHK(hipEventRecord(c->ev[e++], c->stream));
+    return SOLAR_OK;
+  }
   // Every stage also writes its output's edge columns to c->ecols, from which
   // the next gather reads its W/E strips; the first gather reads the state.
   for (int s = 0; s < 5; ++s) {
@@ -293,6 +341,11 @@ int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double
   c->q[0] = q0;
   c->q[1] = q1;
   c->qw = qwb;
+  if (!c->f32 && !env_flag("SOLAR_STEP_STORE_ALL")) {  // the stored-q schedule's second accumulator
+    char* qw2 = nullptr;
+    if ((rc = dev_alloc(c, &qw2, nq * c->esz))) return bail(rc);
+    c->qw2 = qw2;
+  }
   if (!env_flag("SOLAR_NO_EDGE_COLS")) {
     char* ec = nullptr;
     if ((rc = dev_alloc(c, &ec, solar::edge_cols_elems(n, nr) * c->esz))) return bail(rc);
@@ -310,6 +363,8 @@ int solar_init_ex(solar_ctx** out, int n, int nr, const double* gr, const double
   c->shape.stage_tj = env_int("SOLAR_STAGE_TJ");
   c->shape.filter_ti = env_int("SOLAR_FILTER_TI");
   c->shape.filter_tj = env_int("SOLAR_FILTER_TJ");
+  c->shape.adj_stage_chunks = env_int("SOLAR_ADJ_STAGE_CHUNKS");
+  c->shape.adj_filter_chunks = env_int("SOLAR_ADJ_FILTER_CHUNKS");
   if (c->profile) {
     c->ev.resize(13);
     for (auto& e : c->ev)
@@ -604,6 +659,15 @@ int solar_kernel_times(solar_ctx* c, double* out) {
   return SOLAR_OK;
 }
 
+int solar_algorithmic_bytes(solar_ctx* c, double* out) {
+  if (!c || !out) return fail(SOLAR_ERR_ARG, "solar_algorithmic_bytes: null argument");
+  // in states of 5 fields: reads + writes of q and qw per stage, then the filter
+  static const int kStoreAll[6] = {3, 4, 4, 4, 3, 2}, kStoredQ[6] = {2, 4, 3, 4, 3, 2};
+  const int* u = c->qw2 ? kStoredQ : kStoreAll;
+  for (int x = 0; x < 6; ++x) out[x] = static_cast<double>(u[x]) * 5.0 * static_cast<double>(c->esz);
+  return SOLAR_OK;
+}
+
 int solar_device_bytes(solar_ctx* c, size_t* bytes) {
   if (!c || !bytes) return fail(SOLAR_ERR_ARG, "solar_device_bytes: null argument");
   *bytes = c->bytes;
@@ -675,7 +739,7 @@ static int download_f64(solar_ctx* c, double* dst, const double* src, size_t cou
 // to c->gc2 (divp[s] = div(v) of the input of forward stage s, 0-based).
 static int enqueue_adj_step(solar_ctx* c, double dt, double* lam, double* const* divp) {
   const solar::DevTables& T = c->dt;
-  HIPCHK(solar::launch_adj_filter(T, lam, c->tmp, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_filter(T, lam, c->tmp, c->fgx, c->fgy, c->stream, c->shape.adj_filter_chunks));
   HIPCHK(solar::launch_adj_scatter(T, false, c->fgx, c->fgy, nullptr, nullptr, c->tmp, nullptr, 0.0, 0, 5, c->stream,
                                    true));
   for (int s = 5; s >= 1; --s) {
@@ -698,6 +762,7 @@ static int enqueue_adj_step(solar_ctx* c, double dt, double* lam, double* const*
     a.gya_p = c->gyp;
     a.gxa_v = c->gxv;
     a.gya_v = c->gyv;
+    a.chunks = c->shape.adj_stage_chunks;
     HIPCHK(solar::launch_adj_stage(T, a, c->stream));
     HIPCHK(solar::launch_adj_scatter(T, true, c->gxp, c->gyp, c->gxv, c->gyv, lam, a.Wout, a.beta, 4, 1, c->stream));
   }
@@ -842,6 +907,7 @@ int solar_adj_rhs(solar_ctx* c, const double* w, double* out) {
   a.gya_p = c->gyp;
   a.gxa_v = c->gxv;
   a.gya_v = c->gyv;
+  a.chunks = c->shape.adj_stage_chunks;
   HIPCHK(solar::launch_adj_stage(c->dt, a, c->stream));
   HIPCHK(solar::launch_adj_scatter(c->dt, true, c->gxp, c->gyp, c->gxv, c->gyv, c->tmp, nullptr, 0.0, 4, 1, c->stream));
   return download_f64(c, out, c->tmp, nq);
@@ -853,7 +919,7 @@ int solar_adj_filter(solar_ctx* c, const double* o, double* out) {
   if (rc) return rc;
   const size_t nq = c->nf * 30;
   if ((rc = upload_f64(c, c->lam, o, nq))) return rc;
-  HIPCHK(solar::launch_adj_filter(c->dt, c->lam, c->tmp, c->fgx, c->fgy, c->stream));
+  HIPCHK(solar::launch_adj_filter(c->dt, c->lam, c->tmp, c->fgx, c->fgy, c->stream, c->shape.adj_filter_chunks));
   HIPCHK(solar::launch_adj_scatter(c->dt, false, c->fgx, c->fgy, nullptr, nullptr, c->tmp, nullptr, 0.0, 0, 5,
                                    c->stream, true));
   return download_f64(c, out, c->tmp, nq);

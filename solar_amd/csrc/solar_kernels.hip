@@ -63,12 +63,16 @@ __device__ __forceinline__ int strip_index(int e, int l, int a, int n) {
 //     instead of three values from every row of the field.
 //   NI: strip points per thread (3n <= NI * blockDim.x)
 //   sline: 2 buffers x 3 layers x (3 quantities for vectors, else 1) x n
+//   qwr (not null, row strips only): the source is q + betar * qwr, the
+//     input of a stage that is rebuilt from q and qw (the edge columns hold
+//     it as computed).
 template <bool VEC, typename S, int NI>
 __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, int k0, int k1, int fld,
                                             const S* __restrict__ q, const S* __restrict__ ecols, int sfield,
                                             double* __restrict__ gxs, double* __restrict__ gys,
                                             double* __restrict__ gxv, double* __restrict__ gyv,
-                                            long long field_stride_out, double* sline) {
+                                            long long field_stride_out, double* sline, const S* __restrict__ qwr,
+                                            double betar) {
   constexpr int NQ = VEC ? 3 : 1;
   const int n = T.n, nr = T.nr;
   const Route rt = T.route[P * 4 + e];
@@ -82,6 +86,11 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
   const auto sp = to_global(src + (size_t)(sfield + fld) * fstr);
   const auto vx = to_global(src + fstr);
   const auto ve = vx + fstr;
+  const bool rec = qwr != nullptr && !from_cols;  // block-uniform
+  const auto wsrc = to_global((rec ? qwr : q) + (size_t)s * 5 * nf);
+  const auto wsp = wsrc + (size_t)(sfield + fld) * nf;
+  const auto wvx = wsrc + nf;
+  const auto wve = wvx + nf;
   const int ctype = s < 4 ? 0 : s - 3;
   const double* cv = T.conv + (size_t)ctype * plane * 6;
   const int lstride = NQ * n, bstride = 3 * lstride;
@@ -129,7 +138,7 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
   const auto gv = to_global(e < 2 ? gxv : gyv);
 
   // source values of one plane, in the storage type until staged
-  S pv[NI], xv[NI], yv[NI];
+  S pv[NI], xv[NI], yv[NI], pw[NI], xw[NI], yw[NI];
   auto load = [&](int k) {
     const size_t kp = (size_t)k * kstr;
 #pragma unroll
@@ -138,6 +147,13 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
       if (VEC) {
         xv[it] = vx[kp + sidx[it]];
         yv[it] = ve[kp + sidx[it]];
+      }
+      if (rec) {
+        pw[it] = wsp[kp + sidx[it]];
+        if (VEC) {
+          xw[it] = wvx[kp + sidx[it]];
+          yw[it] = wve[kp + sidx[it]];
+        }
       }
     }
   };
@@ -148,9 +164,15 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
     for (int it = 0; it < NI; ++it) {
       if (!sval[it]) continue;
       double* ln = buf + sslot[it];
-      ln[0] = static_cast<double>(pv[it]);
+      double pvd = static_cast<double>(pv[it]);
+      if (rec) pvd = pvd + betar * static_cast<double>(pw[it]);
+      ln[0] = pvd;
       if (VEC) {
-        const double x = static_cast<double>(xv[it]), y = static_cast<double>(yv[it]);
+        double x = static_cast<double>(xv[it]), y = static_cast<double>(yv[it]);
+        if (rec) {
+          x = x + betar * static_cast<double>(xw[it]);
+          y = y + betar * static_cast<double>(yw[it]);
+        }
         ln[n] = c[it][0] * (c[it][1] * x + c[it][2] * y);
         ln[2 * n] = c[it][3] * (c[it][4] * x + c[it][5] * y);
       }
@@ -186,12 +208,13 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, 
                                                  const S* __restrict__ ecols, int sfield,
                                                  double* __restrict__ gxs, double* __restrict__ gys,
                                                  double* __restrict__ gxv, double* __restrict__ gyv,
-                                                 long long field_stride_out) {
+                                                 long long field_stride_out, const S* __restrict__ qwr,
+                                                 double betar) {
   extern __shared__ double sline[];
   const int kc = gather_kc(T.nr);
   const int k0 = (blockIdx.x / 24) * kc, edge = blockIdx.x % 24;
   gather_edge<VEC, S, NI>(T, edge / 4, edge % 4, k0, min(k0 + kc, T.nr), blockIdx.y, q, ecols, sfield, gxs, gys,
-                          gxv, gyv, field_stride_out, sline);
+                          gxv, gyv, field_stride_out, sline, qwr, betar);
 }
 
 // Values one thread loads for one radial plane of the stage kernel: its own
@@ -213,6 +236,8 @@ struct StagePlane {
   double ogp, ogv;             // PART: ghosts of an outside thread's own cell (p' and v_xi or v_eta)
   double c2;                   // optional 3-D c^2 at (i,j,k)
   double tdc2, tdiv;           // tangent-linear inputs at (i,j,k)
+  // RECON: qw at the halo cells and at the plane entering the window
+  S rpx, rvx, rpy0, rve0, rpy1, rve1, wpn, wrn;
 };
 
 // One RK stage (or one RHS evaluation when rhs_mode is set), one pass over
@@ -222,6 +247,14 @@ struct StagePlane {
 // LDS tile, and plane k+1 is loaded into registers while plane k is computed.
 //   KIND 0: first stage (qw taken as 0, written), 1: middle stage (qw read and
 //   written), 2: last stage (qw read, not written back).
+//   XF (FP64 storage, single grid): the stage schedule that stores q after
+//   stages 2, 4 and 5 only.  kStoreNoQ: the stage stores qw (in place) and
+//   the edge columns of q, but not q itself.  kStageRecon (KIND 1): the stage
+//   input is qin + beta_r * qw_a, rebuilt wherever it is read (own point,
+//   halo cells, radial window) with the expression the previous stage
+//   rounded when it computed it, so the same bits; q goes to qout and qw to
+//   qwo_a, a buffer other than qw_a (other blocks still read qw_a's halo
+//   cells).
 // Occupancy floor of the stage kernel (waves per SIMD); 0 leaves it to the
 // compiler.  2 keeps every variant within 256 registers: left alone, the
 // ensemble and tangent-linear middle stages take 254-255 VGPRs plus 2 AGPRs
@@ -258,7 +291,7 @@ struct StagePlane {
 //   of their own cell (E ghosts of p' and v_xi, N ghosts of p' and v_eta)
 //   into the LDS tile, where the in-panel threads' stencils read them.  Every
 //   other cell an outside thread would load reads a fixed in-bounds address.
-template <int TI, int TJ, int KIND, typename S, int MODE = 0, bool PART = false>
+template <int TI, int TJ, int KIND, typename S, int MODE = 0, bool PART = false, int XF = 0>
 __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Targ, const S* __restrict__ qin_a,
                                                    S* __restrict__ qout_a, S* __restrict__ qw_a,
                                                    const double* __restrict__ gxp_a, const double* __restrict__ gyp_a,
@@ -268,7 +301,12 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
                                                    double* __restrict__ divout, int div32,
                                                    const double* __restrict__ tl_dc2, const double* __restrict__ tl_div,
                                                    S* __restrict__ ecols_a, const EnsMember* __restrict__ M,
-                                                   const int4* __restrict__ items, int nitems, int cur) {
+                                                   const int4* __restrict__ items, int nitems, int cur,
+                                                   S* __restrict__ qwo_a, double beta_r) {
+  constexpr bool NOQ = (XF & kStoreNoQ) != 0, RECON = (XF & kStageRecon) != 0;
+  static_assert(XF == 0 || (MODE == 0 && sizeof(S) == 8), "the stage schedule variants are FP64, single grid");
+  static_assert(!RECON || (KIND == 1 && !NOQ), "a rebuilt input is a middle stage's");
+  static_assert(!NOQ || KIND != 2, "the last stage stores q");
   __shared__ double sp[2][TJ + 6][TI + 6];
   __shared__ double svx[2][TJ][TI + 6];
   __shared__ double sve[2][TJ + 6][TI];
@@ -326,6 +364,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const auto f4 = f3 + nf;                     // p'
   const auto o0 = qout + (size_t)(P * 5) * nf;
   const auto w0 = qw + (size_t)(P * 5) * nf;
+  // where qw is stored: in place, or (RECON) in the other accumulator
+  const auto wo0 = RECON ? to_global(qwo_a) + (size_t)(P * 5) * nf : w0;
   // edge-column copy: layer l = i (W) or n-1-i (E); the others store none
   const int ecl = i <= 3 ? i : n - 1 - i;
   const bool ec_on = !ENS && !rhs_mode && ecols_a != nullptr && ecl >= 1 && ecl <= 3 && own_in;
@@ -425,6 +465,15 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
       L.hve0 = hy0_g ? gyvS[kg + hy0_off] : f2[kf + hy0_off];
       L.hpy1 = hy1_g ? gypS[kg + hy1_off] : f4[kf + hy1_off];
       L.hve1 = hy1_g ? gyvS[kg + hy1_off] : f2[kf + hy1_off];
+      if constexpr (RECON) {  // qw of the field cells (a ghost cell reads offset 0 and does not use it)
+        const int fx = hx_g ? 0 : hx_off, fy0 = hy0_g ? 0 : hy0_off, fy1 = hy1_g ? 0 : hy1_off;
+        L.rpx = w0[4 * nf + kf + fx];
+        L.rvx = w0[nf + kf + fx];
+        L.rpy0 = w0[4 * nf + kf + fy0];
+        L.rve0 = w0[2 * nf + kf + fy0];
+        L.rpy1 = w0[4 * nf + kf + fy1];
+        L.rve1 = w0[2 * nf + kf + fy1];
+      }
     } else {
       const int fx = hx_g ? 0 : hx_off, fy0 = hy0_g ? 0 : hy0_off, fy1 = hy1_g ? 0 : hy1_off;
       L.fpx = f4[kf + fx];
@@ -457,6 +506,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     const int kn = min(k + 3, nr - 1);
     L.pn = f4[(size_t)kn * plane + ij];
     L.rn = f3[(size_t)kn * plane + ij];
+    if constexpr (RECON) {
+      L.wpn = w0[4 * nf + (size_t)kn * plane + ij];
+      L.wrn = w0[3 * nf + (size_t)kn * plane + ij];
+    }
     // per-point inputs of the RHS, prefetched with the plane: a load issued
     // in the body would be waited on together with the whole prefetch
     const size_t dx = (size_t)P * nf + kp;
@@ -479,6 +532,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     const int kq = min(kb + q, nr - 1);
     pw[q] = f4[(size_t)kq * plane + ij];
     rw[q] = f3[(size_t)kq * plane + ij];
+    if constexpr (RECON) {
+      pw[q] = pw[q] + beta_r * w0[4 * nf + (size_t)kq * plane + ij];
+      rw[q] = rw[q] + beta_r * w0[3 * nf + (size_t)kq * plane + ij];
+    }
   }
   // The next plane is loaded while this one is computed (a second plane in
   // flight was measured: no gain for FP64 or FP32 storage).  Each iteration
@@ -501,8 +558,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     } else {
 #pragma unroll
       for (int f = 0; f < 5; ++f) {
-        __builtin_nontemporal_store(pq[f], o0 + f * nf + px);
-        if (KIND != 2) __builtin_nontemporal_store(pwv[f], w0 + f * nf + px);
+        if (!NOQ) __builtin_nontemporal_store(pq[f], o0 + f * nf + px);
+        if (KIND != 2) __builtin_nontemporal_store(pwv[f], wo0 + f * nf + px);
       }
       if (ec_on) {
         const size_t x = ec_base + (size_t)pk * edge_cols_plane(n);
@@ -524,8 +581,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
         pw[q] = pw[q + 1];
         rw[q] = rw[q + 1];
       }
-      pw[6] = cur.pn;
-      rw[6] = cur.rn;
+      pw[6] = RECON ? cur.pn + beta_r * cur.wpn : cur.pn;
+      rw[6] = RECON ? cur.rn + beta_r * cur.wrn : cur.rn;
       kb = want;
     }
     // p', v_r at plane k: window slot k - kb (3 on interior planes)
@@ -538,7 +595,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
     const double pc = pcs, rc = rcs;
     const int bf = k & 1;
-    double own_p = pc, own_vx = cur.vx, own_ve = cur.ve;
+    // the stage input at the own point (RECON: rebuilt from q and qw)
+    const double vx = RECON ? cur.vx + beta_r * cur.w[1] : cur.vx;
+    const double ve = RECON ? cur.ve + beta_r * cur.w[2] : cur.ve;
+    const double q1 = RECON ? cur.q1 + beta_r * cur.w[0] : cur.q1;
+    double own_p = pc, own_vx = vx, own_ve = ve;
     if constexpr (PART) {
       if (og == 1) {
         own_p = cur.ogp;
@@ -551,17 +612,32 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     sp[bf][ty + 3][tx + 3] = own_p;
     svx[bf][ty][tx + 3] = own_vx;
     sve[bf][ty + 3][tx] = own_ve;
-    if (hx) {
-      sp[bf][ty + 3][hx_col] = kWide || hx_g ? cur.hpx : static_cast<double>(cur.fpx);
-      svx[bf][ty][hx_col] = kWide || hx_g ? cur.hvx : static_cast<double>(cur.fvx);
-    }
-    if (hy0) {
-      sp[bf][ty][tx + 3] = kWide || hy0_g ? cur.hpy0 : static_cast<double>(cur.fpy0);
-      sve[bf][ty][tx] = kWide || hy0_g ? cur.hve0 : static_cast<double>(cur.fve0);
-    }
-    if (hy1) {
-      sp[bf][ty + 6][tx + 3] = kWide || hy1_g ? cur.hpy1 : static_cast<double>(cur.fpy1);
-      sve[bf][ty + 6][tx] = kWide || hy1_g ? cur.hve1 : static_cast<double>(cur.fve1);
+    if constexpr (RECON) {  // field halo cells rebuilt; ghost cells are the gathered stage input
+      if (hx) {
+        sp[bf][ty + 3][hx_col] = hx_g ? cur.hpx : cur.hpx + beta_r * cur.rpx;
+        svx[bf][ty][hx_col] = hx_g ? cur.hvx : cur.hvx + beta_r * cur.rvx;
+      }
+      if (hy0) {
+        sp[bf][ty][tx + 3] = hy0_g ? cur.hpy0 : cur.hpy0 + beta_r * cur.rpy0;
+        sve[bf][ty][tx] = hy0_g ? cur.hve0 : cur.hve0 + beta_r * cur.rve0;
+      }
+      if (hy1) {
+        sp[bf][ty + 6][tx + 3] = hy1_g ? cur.hpy1 : cur.hpy1 + beta_r * cur.rpy1;
+        sve[bf][ty + 6][tx] = hy1_g ? cur.hve1 : cur.hve1 + beta_r * cur.rve1;
+      }
+    } else {
+      if (hx) {
+        sp[bf][ty + 3][hx_col] = kWide || hx_g ? cur.hpx : static_cast<double>(cur.fpx);
+        svx[bf][ty][hx_col] = kWide || hx_g ? cur.hvx : static_cast<double>(cur.fvx);
+      }
+      if (hy0) {
+        sp[bf][ty][tx + 3] = kWide || hy0_g ? cur.hpy0 : static_cast<double>(cur.fpy0);
+        sve[bf][ty][tx] = kWide || hy0_g ? cur.hve0 : static_cast<double>(cur.fve0);
+      }
+      if (hy1) {
+        sp[bf][ty + 6][tx + 3] = kWide || hy1_g ? cur.hpy1 : static_cast<double>(cur.fpy1);
+        sve[bf][ty + 6][tx] = kWide || hy1_g ? cur.hve1 : static_cast<double>(cur.fve1);
+      }
     }
     __syncthreads();
 
@@ -581,7 +657,6 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
       dvr = dvr + row.w[q] * rw[q];
       dpr = dpr + row.w[q] * pw[q];
     }
-    const double vx = cur.vx, ve = cur.ve, q1 = cur.q1;
 
     // grad.f90:125-133 and :49-57.
     const double div =
@@ -940,10 +1015,10 @@ __global__ __launch_bounds__(gather_threads_max(NI)) void k_gather_ens(const Ens
   const long long fs = 12LL * nr * E.T.n * 3;
   if (VEC)
     gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 4, E.gxp, E.gyp,
-                                  E.gxv, E.gyv, fs, sline);
+                                  E.gxv, E.gyv, fs, sline, nullptr, 0.0);
   else
     gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 0, E.fgx, E.fgy,
-                                   nullptr, nullptr, fs, sline);
+                                   nullptr, nullptr, fs, sline, nullptr, 0.0);
 }
 
 // Non-finite scan and v_r sum of squares; acc[0] counts non-finite values,
@@ -1004,7 +1079,8 @@ static hipError_t gather_lds(K kernel, size_t lds) {
 
 template <bool VEC, typename S, int NI>
 static hipError_t launch_gather_v(const DevTables& T, const S* q, const S* ecols, int sfield, int nfields,
-                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st) {
+                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st, const S* qwr,
+                                  double betar) {
   const long long field_stride = 12LL * T.nr * T.n * 3;
   const int kc = gather_kc(T.nr);
   dim3 grid((unsigned)(24 * ((T.nr + kc - 1) / kc)), (unsigned)nfields);
@@ -1013,35 +1089,40 @@ static hipError_t launch_gather_v(const DevTables& T, const S* q, const S* ecols
   hipError_t e = gather_lds(&k_gather<VEC, S, NI>, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((k_gather<VEC, S, NI>), grid, dim3(nt), lds, st, T, q, ecols, sfield, gxs, gys, gxv, gyv,
-                     field_stride);
+                     field_stride, qwr, betar);
   return hipGetLastError();
 }
 
 template <typename S>
 static hipError_t launch_gather_t(const DevTables& T, const S* q, const S* ecols, bool vec, int sfield, int nfields,
-                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st) {
+                                  double* gxs, double* gys, double* gxv, double* gyv, hipStream_t st, const S* qwr,
+                                  double betar) {
   const bool two = 3 * T.n > 1024;  // n > 341: two strip points per thread
   if (vec)
-    return two ? launch_gather_v<true, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st)
-               : launch_gather_v<true, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st);
-  return two ? launch_gather_v<false, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st)
-             : launch_gather_v<false, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st);
+    return two ? launch_gather_v<true, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st, qwr, betar)
+               : launch_gather_v<true, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st, qwr, betar);
+  return two ? launch_gather_v<false, S, 2>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st, qwr, betar)
+             : launch_gather_v<false, S, 1>(T, q, ecols, sfield, nfields, gxs, gys, gxv, gyv, st, qwr, betar);
 }
 
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
-                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols) {
-  if (f32)
+                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols, const void* qwr,
+                         double betar) {
+  if (f32) {
+    if (qwr) return hipErrorInvalidValue;  // rebuilt stage inputs: FP64 storage only
     return launch_gather_t(T, static_cast<const float*>(q), static_cast<const float*>(ecols), vec, sfield, nfields,
-                           gxs, gys, gxv, gyv, st);
+                           gxs, gys, gxv, gyv, st, static_cast<const float*>(nullptr), 0.0);
+  }
   return launch_gather_t(T, static_cast<const double*>(q), static_cast<const double*>(ecols), vec, sfield, nfields,
-                         gxs, gys, gxv, gyv, st);
+                         gxs, gys, gxv, gyv, st, static_cast<const double*>(qwr), betar);
 }
 
 template <int TI, int TJ, typename S, bool PART = false>
 static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp,
                                  const double* gyp, const double* gxv, const double* gyv, int stage, double dt,
                                  int rhs_mode, hipStream_t st, const LaunchShape& ls, const double* c2f,
-                                 double* divout, int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
+                                 double* divout, int div32, const double* tl_dc2, const double* tl_div, S* ecols,
+                                 int xf, S* qwo, double beta_r) {
   const int tiles = ((T.n + TI - 1) / TI) * ((T.n + TJ - 1) / TJ) * 6;
   // radial chunks of >= 32 planes, ~12k blocks: measured best at 6x128^2x256 and 6x256^2x384
   const int nkch = ls.stage_chunks > 0 ? ls.stage_chunks : chunks_for(tiles, T.nr, 12288, 32);
@@ -1049,26 +1130,45 @@ static hipError_t launch_stage_t(const DevTables& T, const S* qin, S* qout, S* q
   dim3 grid(tiles * nkch), block(TI, TJ);
   const double al = kAlphaRk4[stage < 0 ? 0 : stage], be = kBetaRk4[stage < 0 ? 0 : stage];
   const int kind = (rhs_mode || stage == 0) ? 0 : (stage < 4 ? 1 : 2);
-#define SOLAR_STAGE_LAUNCH(K, M)                                                                                  \
-  hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M, PART>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv, al, \
-                     be, dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, ecols, nullptr, nullptr, 0, 0)
+#define SOLAR_STAGE_LAUNCH(K, M, X)                                                                                  \
+  hipLaunchKernelGGL((k_stage<TI, TJ, K, S, M, PART, X>), grid, block, 0, st, T, qin, qout, qw, gxp, gyp, gxv, gyv,  \
+                     al, be, dt, kchunk, nkch, rhs_mode, c2f, divout, div32, tl_dc2, tl_div, ecols, nullptr, nullptr, \
+                     0, 0, qwo, beta_r)
+  if (xf) {
+    // stored-q schedule variants: FP64, 32x8 and 16x16 tiles, no RHS or tangent-linear mode
+    constexpr bool kHave = sizeof(S) == 8 && ((TI == 32 && TJ == 8) || (TI == 16 && TJ == 16));
+    if constexpr (kHave) {
+      if (rhs_mode || tl_dc2) return hipErrorInvalidValue;
+      if (xf == kStoreNoQ && kind == 0)
+        SOLAR_STAGE_LAUNCH(0, 0, kStoreNoQ);
+      else if (xf == kStoreNoQ && kind == 1)
+        SOLAR_STAGE_LAUNCH(1, 0, kStoreNoQ);
+      else if (xf == kStageRecon && kind == 1 && qwo)
+        SOLAR_STAGE_LAUNCH(1, 0, kStageRecon);
+      else
+        return hipErrorInvalidValue;
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   if (tl_dc2) {
     if constexpr (sizeof(S) == 8 && !PART) {
       if (kind == 0)
-        SOLAR_STAGE_LAUNCH(0, 2);
+        SOLAR_STAGE_LAUNCH(0, 2, 0);
       else if (kind == 1)
-        SOLAR_STAGE_LAUNCH(1, 2);
+        SOLAR_STAGE_LAUNCH(1, 2, 0);
       else
-        SOLAR_STAGE_LAUNCH(2, 2);
+        SOLAR_STAGE_LAUNCH(2, 2, 0);
     } else {
       return hipErrorInvalidValue;  // the tangent-linear stage is FP64 only, on tiles that divide n
     }
   } else if (kind == 0) {
-    SOLAR_STAGE_LAUNCH(0, 0);
+    SOLAR_STAGE_LAUNCH(0, 0, 0);
   } else if (kind == 1) {
-    SOLAR_STAGE_LAUNCH(1, 0);
+    SOLAR_STAGE_LAUNCH(1, 0, 0);
   } else {
-    SOLAR_STAGE_LAUNCH(2, 0);
+    SOLAR_STAGE_LAUNCH(2, 0, 0);
   }
 #undef SOLAR_STAGE_LAUNCH
   return hipGetLastError();
@@ -1086,23 +1186,23 @@ template <typename S>
 static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* qw, const double* gxp, const double* gyp,
                                  const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                                  hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout,
-                                 int div32, const double* tl_dc2, const double* tl_div, S* ecols) {
+                                 int div32, const double* tl_dc2, const double* tl_div, S* ecols, int xf, S* qwo,
+                                 double beta_r) {
+#define SOLAR_STAGE_ARGS \
+  T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div, ecols, xf, qwo, beta_r
   if (T.n % 16 != 0) {  // partial tiles: 32x8 or 16x16
-    if (pad_pick_ti(T.n) == 32)
-      return launch_stage_t<32, 8, S, true>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f,
-                                            divout, div32, tl_dc2, tl_div, ecols);
-    return launch_stage_t<16, 16, S, true>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f,
-                                           divout, div32, tl_dc2, tl_div, ecols);
+    if (pad_pick_ti(T.n) == 32) return launch_stage_t<32, 8, S, true>(SOLAR_STAGE_ARGS);
+    return launch_stage_t<16, 16, S, true>(SOLAR_STAGE_ARGS);
   }
   int ti = ls.stage_ti, tj = ls.stage_tj;
-  if (!fits(T.n, ti, tj)) {  // measured best: 32x8 (64x8 within 2%), then 16x16
+  // measured best: 32x8 (64x8 within 2%), then 16x16; the stored-q schedule
+  // variants exist for these two only
+  if (!fits(T.n, ti, tj) || (xf && !((ti == 32 && tj == 8) || (ti == 16 && tj == 16)))) {
     ti = pick_ti(T.n);
     tj = ti == 64 ? 8 : 256 / ti;
   }
 #define SOLAR_STAGE_CASE(A, B) \
-  if (ti == A && tj == B)      \
-    return launch_stage_t<A, B, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, \
-                                   div32, tl_dc2, tl_div, ecols);
+  if (ti == A && tj == B) return launch_stage_t<A, B, S>(SOLAR_STAGE_ARGS);
   SOLAR_STAGE_CASE(64, 8)
   SOLAR_STAGE_CASE(64, 4)
   SOLAR_STAGE_CASE(32, 8)
@@ -1111,27 +1211,25 @@ static hipError_t launch_stage_s(const DevTables& T, const S* qin, S* qout, S* q
   SOLAR_STAGE_CASE(32, 4)
 #undef SOLAR_STAGE_CASE
   ti = pick_ti(T.n);
-  if (ti == 64)
-    return launch_stage_t<64, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32, tl_dc2, tl_div, ecols);
-  if (ti == 32)
-    return launch_stage_t<32, 8, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                    div32, tl_dc2, tl_div, ecols);
-  return launch_stage_t<16, 16, S>(T, qin, qout, qw, gxp, gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout,
-                                   div32, tl_dc2, tl_div, ecols);
+  if (ti == 64) return launch_stage_t<64, 8, S>(SOLAR_STAGE_ARGS);
+  if (ti == 32) return launch_stage_t<32, 8, S>(SOLAR_STAGE_ARGS);
+  return launch_stage_t<16, 16, S>(SOLAR_STAGE_ARGS);
+#undef SOLAR_STAGE_ARGS
 }
 
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls, const double* c2f, double* divout, int div32,
-                        const double* tl_dc2, const double* tl_div, void* ecols) {
-  if (f32)
+                        const double* tl_dc2, const double* tl_div, void* ecols, int xf, void* qwo, double beta_r) {
+  if (f32) {
+    if (xf) return hipErrorInvalidValue;  // the stored-q schedule variants are FP64
     return launch_stage_s(T, static_cast<const float*>(qin), static_cast<float*>(qout), static_cast<float*>(qw), gxp,
                           gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div,
-                          static_cast<float*>(ecols));
+                          static_cast<float*>(ecols), 0, static_cast<float*>(nullptr), 0.0);
+  }
   return launch_stage_s(T, static_cast<const double*>(qin), static_cast<double*>(qout), static_cast<double*>(qw), gxp,
                         gyp, gxv, gyv, stage, dt, rhs_mode, st, ls, c2f, divout, div32, tl_dc2, tl_div,
-                        static_cast<double*>(ecols));
+                        static_cast<double*>(ecols), xf, static_cast<double*>(qwo), beta_r);
 }
 
 template <int TI, int TJ, typename S, bool PART = false>
@@ -1210,15 +1308,15 @@ static hipError_t launch_stage_ens_t(const EnsMember* M, const int4* items, int 
   if (stage == 0)
     hipLaunchKernelGGL((k_stage<TI, TJ, 0, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
-                       nitems, cur);
+                       nitems, cur, nullptr, 0.0);
   else if (stage < 4)
     hipLaunchKernelGGL((k_stage<TI, TJ, 1, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
-                       nitems, cur);
+                       nitems, cur, nullptr, 0.0);
   else
     hipLaunchKernelGGL((k_stage<TI, TJ, 2, double, 1>), g, block, 0, st, none, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, nullptr, al, be, 0.0, 0, 0, 0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, M, items,
-                       nitems, cur);
+                       nitems, cur, nullptr, 0.0);
   return hipGetLastError();
 }
 

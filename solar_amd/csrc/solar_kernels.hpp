@@ -45,13 +45,18 @@ struct LaunchShape {
   int filter_chunks = 0;
   int stage_ti = 0, stage_tj = 0;
   int filter_ti = 0, filter_tj = 0;
+  int adj_stage_chunks = 0, adj_filter_chunks = 0;  // sensitivity kernels
 };
 
 // State pointers are float* when f32 is set (FP32-state variant), double* otherwise.
 // ecols (optional): the edge_cols copies of q's W/E edge columns; the
 // column strips are then read from there.
+// qwr (optional, FP64 storage): the gathered field is q + betar * qwr, a
+// stage input rebuilt from q and qw (kStageRecon); ecols then holds it as
+// computed.
 hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, int sfield, int nfields, double* gxs,
-                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols = nullptr);
+                         double* gys, double* gxv, double* gyv, hipStream_t st, const void* ecols = nullptr,
+                         const void* qwr = nullptr, double betar = 0.0);
 // c2f (optional): a 3-D c^2 field (n,n,nr,6) used instead of the radial
 // profile; divout (optional): receives div(v) of the stage input (n,n,nr,6),
 // as double, or as float when div32 is set.  tl_dc2, tl_div (optional, both
@@ -59,11 +64,18 @@ hipError_t launch_gather(const DevTables& T, const void* q, bool f32, bool vec, 
 // pressure equation (dc2 and div_primal fields (n,n,nr,6)).  ecols
 // (optional, not with rhs_mode): receives the output's edge columns in the
 // edge_cols layout (p', v_xi, v_eta; all five fields after the last stage).
+// xf (FP64 storage, not with rhs_mode or tl_dc2): kStoreNoQ, the stage
+// stores qw in place and the edge columns of its output, not q (qout unused);
+// kStageRecon (stages 2-4), the stage input is qin + beta_r * qw, its qw goes
+// to qwo (not qw) and its q to qout.  Tile shapes other than 32x8 and 16x16
+// are not compiled for these: requested ones fall back to the default.
+constexpr int kStoreNoQ = 1, kStageRecon = 2;
 hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* qw, bool f32, const double* gxp,
                         const double* gyp, const double* gxv, const double* gyv, int stage, double dt, int rhs_mode,
                         hipStream_t st, const LaunchShape& ls, const double* c2f = nullptr,
                         double* divout = nullptr, int div32 = 0, const double* tl_dc2 = nullptr,
-                        const double* tl_div = nullptr, void* ecols = nullptr);
+                        const double* tl_div = nullptr, void* ecols = nullptr, int xf = 0, void* qwo = nullptr,
+                        double beta_r = 0.0);
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
                          const double* fgy, hipStream_t st, const LaunchShape& ls);
 // Ensemble launches.  items: 4 ints per work item, (member, bi | bj << 16,

@@ -45,6 +45,7 @@ EXPORTS = [
     ("solar_norm_vr", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_kernel_times", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]),
+    ("solar_algorithmic_bytes", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_set_c2_field", ctypes.c_int, [ctypes.c_void_p, _D]),
     ("solar_set_adjoint_options", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("solar_set_adjoint_cache", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -361,6 +362,13 @@ class Solver:
         _check(self.lib.solar_kernel_times(self.h, _p(out)), "solar_kernel_times")
         return dict(stage_ms=out[0], stage_gather_ms=out[1], filter_ms=out[2], filter_gather_ms=out[3],
                     step_ms=out[4])
+
+    def algorithmic_bytes(self):
+        """Algorithmic HBM bytes per grid point of each launch of a step:
+        (stage 1, ..., stage 5, filter)."""
+        out = np.zeros(6)
+        _check(self.lib.solar_algorithmic_bytes(self.h, _p(out)), "solar_algorithmic_bytes")
+        return tuple(float(x) for x in out)
 
     def device_bytes(self):
         v = ctypes.c_size_t()

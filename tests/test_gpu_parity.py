@@ -186,6 +186,47 @@ def test_gathers_from_state_and_from_edge_columns_agree(t0, monkeypatch, bits):
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("n,nr,edge_cols", [(16, 24, "0"), (48, 20, "0"), (64, 24, "0"), (20, 16, "0"),
+                                             (64, 24, "1")])
+def test_stored_q_schedule_equals_storing_every_stage(n, nr, edge_cols, monkeypatch):
+    """An FP64 step stores q after stages 2, 4 and 5 only: stages 1 and 3
+    store their accumulator and the edge columns of q, and stages 2 and 4
+    (and the gathers before them) rebuild their input as q + beta*qw.  The
+    same steps storing q after every stage (SOLAR_STEP_STORE_ALL=1) give the
+    same bits: 16x16 tiles (n = 16, 48), 32x8 (n = 64), partial tiles
+    (n = 20), gathers from the state (SOLAR_NO_EDGE_COLS=1), and the stage
+    divergences and gradient of the sensitivity path."""
+    from solar_amd import background, generate_ic
+    monkeypatch.setenv("SOLAR_NO_EDGE_COLS", edge_cols)
+    bg = background(1, nr)
+    L = max(1, round(0.08 * nr))
+    q0, _ = generate_ic(1, n, nr, bg[0], bg[1], 11, [1e-3, 1e-6, 8])
+    q0 = q0 + 1e-4 * np.random.default_rng(n + nr).standard_normal(q0.shape)
+    dt = 0.5 * min(bg[0][1] - bg[0][0], bg[0][0] * (np.pi / 2) / (n - 1))
+    rec = np.array([[p, nr - 3, (7 * p) % n, (3 * p + 1) % n] for p in range(6)])
+    print(f"[case] schedule n={n} nr={nr} no_edge_cols={edge_cols}", flush=True)
+    runs = []
+    for store_all in ("0", "1"):
+        monkeypatch.setenv("SOLAR_STEP_STORE_ALL", store_all)
+        with Solver(n, nr, *bg, L, 0.1) as s:
+            s.set_state(q0)
+            s.rk4(3, dt)
+            state = s.get_state()
+            grad = None
+            if n % 16 == 0:  # the sensitivity kernels tile the panel exactly
+                s.set_receivers(rec)
+                s.set_state(q0)
+                data = 0.9 * s.record(6, dt)
+                _, grad, _ = s.misfit_gradient(q0, 6, dt, data, checkpoint_every=3)
+            runs.append((state, grad, s.device_bytes()))
+    (a, ga, ba), (b, gb, bb) = runs
+    print(f"[schedule] n={n}: differing states {int(np.count_nonzero(a != b))}; device bytes {ba} vs {bb}",
+          flush=True)
+    assert np.array_equal(a, b)
+    assert ga is None or np.array_equal(ga, gb)
+    assert ba > bb  # the second accumulator is allocated only for the stored-q schedule
+
+
 @pytest.mark.parametrize("n,nr,kind,steps", [(48, 20, 1, 2), (64, 32, 1, 2), (128, 24, 0, 1)])
 def test_rk4_tile_shapes(n, nr, kind, steps):
     """Every tile shape the launcher picks (16x16, 32x8, 64x4) and chunked radial marching."""
