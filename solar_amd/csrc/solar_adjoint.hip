@@ -860,9 +860,10 @@ static int adj_ti(int n) { return (n % 32 == 0) ? 32 : 16; }
 template <int TI, int TJ, bool D32>
 static hipError_t launch_adj_stage_t(const DevTables& T, const AdjStageArgs& a, hipStream_t st) {
   const int tiles = (T.n / TI) * (T.n / TJ) * 6;
-  // about 2k blocks in chunks of >= 24 planes (measured at 6x96^2x192:
-  // 8 chunks 468 us, 6 chunks 479 us, 10 chunks 490 us, 3 chunks 564 us)
-  int nkch = (2048 + tiles - 1) / tiles;
+  // chunks of >= 24 planes, as many as fit in three rounds of the GPU's blocks
+  // (two 4-wave blocks per CU, 256 CUs).  Measured at 6x96^2x192 (216 tiles),
+  // adjoint step: 7 chunks 3.02 ms, 9 3.08, 8 3.12, 16 3.15, 6 3.19, 5 3.38.
+  int nkch = 1536 / tiles;
   const int maxk = T.nr / 24 > 0 ? T.nr / 24 : 1;
   if (nkch > maxk) nkch = maxk;
   if (a.chunks > 0) nkch = std::min(a.chunks, T.nr);
