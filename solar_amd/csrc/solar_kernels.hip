@@ -959,8 +959,17 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   if (kk < kend) step(kk, pa, Store());
 }
 
+// Occupancy floor: six waves per SIMD (80 registers), three 512-thread
+// blocks per CU.  Left alone, the FP32-state filter takes 84 registers and
+// runs five waves, two blocks per CU, which leaves the barriers of each
+// plane less to overlap with; at 80 it does not spill (the 32x16 and 16x16
+// tiles; the other shapes spill 36 bytes and keep no floor).
+template <int TI, int TJ, typename S>
+constexpr int filter_waves() { return (sizeof(S) == 4 && TI * TJ >= 256 && !(TI == 64 && TJ == 4) && !(TI == 32 && TJ == 8)) ? 6 : 1; }
+
 template <int TI, int TJ, typename S, bool PART = false>
-__global__ __launch_bounds__(TI* TJ) void k_filter(DevTables T, const S* __restrict__ qin,
+__global__ __launch_bounds__(TI* TJ) __attribute__((amdgpu_waves_per_eu(filter_waves<TI, TJ, S>())))
+void k_filter(DevTables T, const S* __restrict__ qin,
                                                     S* __restrict__ qout, const double* __restrict__ fgx,
                                                     const double* __restrict__ fgy, int kchunk, int nkch) {
   __shared__ FilterLds<TI, TJ> lds;
