@@ -738,8 +738,8 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
 // window.
 template <int TI, int TJ>
 struct FilterLds {
-  double sq[TJ + 6][TI + 6];
-  double sx[TJ + 6][TI];
+  alignas(16) double sq[TJ + 6][TI + 6];  // rows of TI + 6 doubles: 16-byte aligned pairs
+  alignas(16) double sx[TJ + 6][TI];
 };
 
 // The tile (bi, bj) of field F of panel P (PF = 5P + F) over planes [k0, k1).
@@ -867,6 +867,58 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     return kind[c] ? v : 0.0;
   };
 
+  // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
+  // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
+  // FP32 state: a task is two adjacent outputs of one row, read as four
+  // aligned 16-byte pairs (eight values for two 7-tap sums), 22 x 16 tasks
+  // for a 32 x 16 tile: half the LDS reads of one output per thread (filter
+  // at c3 2.79 -> 2.61 ms).  The FP64 filter, bound by HBM at c3, runs one
+  // output per thread: the pairs made it slower there (3.02 -> 3.15 ms).
+  auto xi_pass = [&]() {
+    if constexpr (kWide) {
+      for (int rr = ty; rr < TJ + 6; rr += TJ) {
+        const int gj = j0 - 3 + rr;
+        double xv;
+        if (gj >= 0 && gj < n) {
+          xv = 0.0;
+#pragma unroll
+          for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
+        } else {
+          xv = sq[rr][tx + 3];
+        }
+        sx[rr][tx] = xv;
+      }
+      return;
+    }
+    constexpr int NTASK = (TJ + 6) * (TI / 2);
+    for (int t = tid; t < NTASK; t += NT) {
+      const int rr = t / (TI / 2), c0 = 2 * (t % (TI / 2));
+      const double2* src = reinterpret_cast<const double2*>(&sq[rr][c0]);
+      double v[8];
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const double2 d = src[h];
+        v[2 * h] = d.x;
+        v[2 * h + 1] = d.y;
+      }
+      const int gj = j0 - 3 + rr;
+      double x0, x1;
+      if (gj >= 0 && gj < n) {
+        x0 = 0.0;
+        x1 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+          x0 = x0 + T.filt[q] * v[q];
+          x1 = x1 + T.filt[q] * v[q + 1];
+        }
+      } else {
+        x0 = v[3];
+        x1 = v[4];
+      }
+      *reinterpret_cast<double2*>(&sx[rr][c0]) = make_double2(x0, x1);
+    }
+  };
+
   if (twod) {  // the one plane: FILTER_XI, FILTER_ETA, stored as they come
     FPlane p0;
     load_plane(0, p0);
@@ -876,18 +928,7 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
       if (e < NCELL) (&sq[0][0])[e] = widen(p0, c);
     }
     __syncthreads();
-    for (int rr = ty; rr < TJ + 6; rr += TJ) {
-      const int gj = j0 - 3 + rr;
-      double xv;
-      if (gj >= 0 && gj < n) {
-        xv = 0.0;
-#pragma unroll
-        for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
-      } else {
-        xv = sq[rr][tx + 3];
-      }
-      sx[rr][tx] = xv;
-    }
+    xi_pass();
     __syncthreads();
     double v = 0.0;
 #pragma unroll
@@ -910,20 +951,7 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     }
     load_plane(min(kk + 2, kend - 1), cur);  // past the end: a plane that is loaded and not used
     __syncthreads();
-    // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
-    // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
-    for (int rr = ty; rr < TJ + 6; rr += TJ) {
-      const int gj = j0 - 3 + rr;
-      double xv;
-      if (gj >= 0 && gj < n) {
-        xv = 0.0;
-#pragma unroll
-        for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
-      } else {
-        xv = sq[rr][tx + 3];
-      }
-      sx[rr][tx] = xv;
-    }
+    xi_pass();
     __syncthreads();
     // FILTER_ETA (filter.f90:448-629).
     double v = 0.0;
