@@ -263,6 +263,14 @@ int solar::upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allo
   int rc;
   d->n = t.n;
   d->nr = t.nr;
+  // the stage divides by C, D, r and rho0 through their reciprocals when all
+  // lie in [2^-60, 2^60] (quotients of normal numerators then stay normal)
+  auto in_range = [](double v) { return v >= 0x1p-60 && v <= 0x1p60; };
+  bool fast = true;
+  for (double v : t.metric_c) fast = fast && in_range(v);
+  for (double v : t.metric_d) fast = fast && in_range(v);
+  for (const auto& row : t.radial) fast = fast && in_range(row.r) && in_range(row.rho0);
+  d->fast_div = fast ? 1 : 0;
   if ((rc = put(&d->ci, t.metric_c)) || (rc = put(&d->dj, t.metric_d)) || (rc = put(&d->dc, t.dc)) ||
       (rc = put(&d->tdx, t.tdx)) || (rc = put(&d->tde, t.tde)) || (rc = put(&d->xy, t.xy)) ||
       (rc = put(&d->rad, t.radial)) || (rc = put(&d->conv, t.conv)) || (rc = put(&d->gw, t.gather_w)) ||

@@ -32,6 +32,28 @@ namespace solar {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// a / b rounded to nearest -- the IEEE quotient, bit for bit -- from
+// y = RN(1/b), precomputed where b is fixed: q = RN(a y), then two
+// corrections q <- RN(q - r y) with the negated remainder r = b q - a, exact
+// by FMA once q is within an ulp of a/b (Markstein: with y within half an ulp
+// of 1/b and q within an ulp of a/b, the correction rounds to RN(a/b)).  The
+// negated remainder keeps the sign of a zero quotient (-0 / b = -0).  The
+// remainders and quotients must stay normal: with the divisors in
+// [2^-60, 2^60] (fast, checked on the host) a numerator whose exponent lies
+// in [-900, 900], or zero, takes this path; others (subnormal field values
+// occur in the tails of Gaussian pulses) take the hardware division, as do
+// all when fast is false.  An infinite numerator gives NaN here where the
+// division gives an infinity (both non-finite).  Five FP64 operations and an
+// exponent test, against about a dozen with a quarter-rate reciprocal.
+__device__ __forceinline__ double div_by(double a, double b, double y, bool fast) {
+  if (__builtin_expect(!fast || (unsigned)(__builtin_amdgcn_frexp_exp(a) + 900) > 1800u, 0)) return a / b;
+  const double q0 = a * y;
+  const double r0 = __builtin_fma(b, q0, -a);
+  const double q1 = __builtin_fma(-r0, y, q0);
+  const double r1 = __builtin_fma(b, q1, -a);
+  return __builtin_fma(-r1, y, q1);
+}
+
 // Source point (i,j) of along-edge index a on the strip of edge e at layer l
 // (1..3): S rows l, N rows n-1-l, W columns l, E columns n-1-l (0-based),
 // i.e. rows/columns 2..4 and N-1..N-3 of grad.f90:28-31.
@@ -522,6 +544,12 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   };
 
   const double Ci = to_global(T.ci)[ic], Dj = to_global(T.dj)[jc];
+  const double yCi = 1.0 / Ci, yDj = 1.0 / Dj;  // the divisors fixed for this thread's column
+  // Reciprocal-based quotients in the FP32-state stages only: c3 stage 2.87 ->
+  // 2.74 ms.  In the FP64 stages the rare-path branches made the compiler
+  // drain the prefetch at the loop's back edge (vmcnt(0)), and the stages got
+  // slower (4.48 -> 4.51 ms at c3); they divide in hardware.
+  const bool fdiv = sizeof(S) == 4 && T.fast_div != 0;
   const double dc = to_global(T.dc)[ij], tdx = to_global(T.tdx)[ij], tde = to_global(T.tde)[ij],
                xy = to_global(T.xy)[ij];
 
@@ -660,18 +688,18 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
 
     // grad.f90:125-133 and :49-57.
-    const double div =
-        (row.inv_r * ((dc * (dvx / Ci + dve / Dj) - tdx * vx) - tde * ve) + dvr) + (2.0 * rc) / row.r;
-    const double gxi = row.inv_r * (Dj * dpx + (xy * dpe) / Dj);
-    const double get = row.inv_r * (Ci * dpe + (xy * dpx) / Ci);
+    const double div = (row.inv_r * ((dc * (div_by(dvx, Ci, yCi, fdiv) + div_by(dve, Dj, yDj, fdiv)) - tdx * vx) - tde * ve) +
+                        dvr) + div_by(2.0 * rc, row.r, row.inv_r, fdiv);
+    const double gxi = row.inv_r * (Dj * dpx + div_by(xy * dpe, Dj, yDj, fdiv));
+    const double get = row.inv_r * (Ci * dpe + div_by(xy * dpx, Ci, yCi, fdiv));
 
     // compute_rhs_acoustic.f90:38-149 (forcing omitted).
     double r1 = -(row.rho0 * div);
     r1 = r1 - rc * row.drho0dr;
-    double r2 = -(gxi / row.rho0);
-    double r3 = -(get / row.rho0);
-    double r4 = -(dpr / row.rho0);
-    r4 = r4 - (q1 * row.g0) / row.rho0;
+    double r2 = -div_by(gxi, row.rho0, row.inv_rho0, fdiv);
+    double r3 = -div_by(get, row.rho0, row.inv_rho0, fdiv);
+    double r4 = -div_by(dpr, row.rho0, row.inv_rho0, fdiv);
+    r4 = r4 - div_by(q1 * row.g0, row.rho0, row.inv_rho0, fdiv);
     // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
     const double c2rho0 = c2f ? cur.c2 * row.rho0 : row.c2rho0;
     const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;

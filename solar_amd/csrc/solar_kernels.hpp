@@ -26,6 +26,7 @@ struct DevTables {
   int inv_route[24];     // (receiver P*4+e) | reverse << 8 per (sender, sender edge)
   double fcx[7], fcy[7];
   double filt[7], filtr[11];
+  int fast_div;  // every divisor of the stage (C, D, r, rho0) lies in [2^-60, 2^60]
 };
 
 // One member of a batched ensemble (FP64): its tables, state ping-pong

@@ -39,7 +39,7 @@ struct RadialRow {
   double drho0dr;  // DDR(rho0)                    :51
   double sigma;    // sponge coefficient           sponge.f90:40-46
   double w[7];     // DDR weights on planes base..base+6 (derivative.f90:656-754)
-  double inv_rho0;  // 1/rho0(k) (used by the adjoint only)
+  double inv_rho0;  // 1/rho0(k), correctly rounded: the divisor of the stage's quotients by rho0
 };
 static_assert(sizeof(RadialRow) == 16 * sizeof(double), "RadialRow layout");
 
