@@ -226,8 +226,13 @@ __device__ __forceinline__ void gather_edge(const DevTables& T, int P, int e, in
 
 // blockIdx.x = (chunk of planes)*24 + (receiver panel P)*4 + (receiver edge e);
 // blockIdx.y selects a field offset (the filter's five scalars).
+// The FP32-state vector gather at six waves per SIMD (80 registers, a 12-byte
+// spill): c3 80 against 97 us.  The FP64 one gained nothing at c3 and lost
+// at 6x96^2x192 (28 against 23 us) and keeps the compiler's choice.
 template <bool VEC, typename S, int NI>
-__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather(DevTables T, const S* __restrict__ q,
+__global__ __launch_bounds__(gather_threads_max(NI))
+__attribute__((amdgpu_waves_per_eu(VEC && NI == 1 && sizeof(S) == 4 ? 6 : 1)))
+void k_gather(DevTables T, const S* __restrict__ q,
                                                  const S* __restrict__ ecols, int sfield,
                                                  double* __restrict__ gxs, double* __restrict__ gys,
                                                  double* __restrict__ gxv, double* __restrict__ gyv,
