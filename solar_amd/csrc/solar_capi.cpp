@@ -291,6 +291,9 @@ int solar::upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allo
     d->filt[q] = solar::kFiltCoeffs[q];
   }
   for (int q = 0; q < 11; ++q) d->filtr[q] = solar::kFiltCoeffsR[q];
+  // the filter kernel computes each radial product once for taps q and 10 - q
+  for (int q = 0; q < 11; ++q)
+    if (d->filtr[q] != d->filtr[10 - q]) return fail(SOLAR_ERR_ARG, "radial filter taps are not symmetric");
   return SOLAR_OK;
 }
 

@@ -813,41 +813,37 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   const int a = twod ? 0 : max(k0, kNstr), bnd = twod ? 1 : min(k1, nr - kNstr);
   if (a >= bnd) return;
 
-  // Tile cells this thread fills: kind 1 a field cell (fp, plane stride
-  // n*n), 2 a ghost-strip cell (gp, plane stride 3n), 0 none (the corner
-  // cells beyond the panel in both directions are never read; they and the
-  // thread slots past the tile stay 0).  Every pointer is valid and every
-  // load is issued unconditionally (unused ones read a fixed in-bounds
-  // address with stride 0), so the loads carry no branches and the waits
-  // before their use count only what is still needed.
+  // Tile cells this thread fills: kind 1 a field cell, 2 a ghost-strip cell,
+  // 0 none.  Field cells load from the plane base (a wave-uniform address)
+  // plus a per-lane 32-bit byte offset, so advancing a plane costs no vector
+  // instruction; ghost cells (plane stride 3n, uniform too) have per-lane
+  // pointers.  Every load is issued unconditionally from a valid address:
+  // lanes without a field cell read the plane's first element, lanes without
+  // a ghost cell the strip's first element.  Kind-0 cells (the corners beyond
+  // the panel in both directions, cells past a partial tile's ghost layers)
+  // feed only outputs that are not stored, so they keep whatever was loaded.
   const int tid = ty * TI + tx;
-  const S* fp[NC];
+  unsigned foff[NC];
   const double* gp[NC];
-  unsigned fst[NC], gst[NC];  // plane strides: n*n*nr < 2^32 for any grid that fits in HBM
   int kind[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int e = tid + c * NT;
     kind[c] = 0;
-    fp[c] = g;
+    foff[c] = 0;
     gp[c] = gxW;
-    fst[c] = 0;
-    gst[c] = 0;
     if (e < NCELL) {
       const int rr = e / (TI + 6), cc = e % (TI + 6);
       const int gj = j0 - 3 + rr, gi = i0 - 3 + cc;
       const bool jin = gj >= 0 && gj < n, iin = gi >= 0 && gi < n;
       if (jin && iin) {
-        fp[c] = g + (size_t)gj * n + gi;
-        fst[c] = (unsigned)plane;
+        foff[c] = (unsigned)(((size_t)gj * n + gi) * sizeof(S));  // < n*n*sizeof(S)
         kind[c] = 1;
       } else if (jin && gi < n + 3) {  // gi >= -3 always; beyond the E ghosts: none
         gp[c] = (gi < 0) ? gxW + (size_t)gj * 3 + (-gi - 1) : gxW + gx_side + (size_t)gj * 3 + (gi - n);
-        gst[c] = 3u * n;
         kind[c] = 2;
       } else if (iin && gj < n + 3) {
         gp[c] = (gj < 0) ? gyS + (size_t)(-gj - 1) * n + gi : gyS + gy_side + (size_t)(gj - n) * n + gi;
-        gst[c] = 3u * n;
         kind[c] = 2;
       }
     }
@@ -856,66 +852,79 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   // whole by the LDS write that starts its step, and its buffer is refilled
   // with the plane two ahead right there, so a load has the rest of its step
   // and the whole next step to arrive (no register copies between
-  // iterations, which would wait on the loads).  With an FP32 state the field
-  // cells stay FP32 in flight (.f) and are widened when they go to LDS, next
-  // to the FP64 ghost-strip cells (.g): a conversion at the load would also
-  // wait on every load in flight.
+  // iterations, which would wait on the loads).  Field cells stay in the
+  // storage type in flight (.f) and are widened when they go to LDS, next to
+  // the FP64 ghost-strip cells (.g): a conversion at the load would also wait
+  // on every load in flight.
+  // FP64 state: one value per cell, from the field or (in a wave with ghost
+  // cells, by a per-lane address select) from the strip.
   constexpr bool kWide = sizeof(S) == sizeof(double);
-  struct FPlane {
+  struct FPlaneN {
     double g[NC];
     S f[NC];
   };
-  if constexpr (kWide) {  // one FP64 source per cell
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-      if (kind[c] == 1) {
-        gp[c] = reinterpret_cast<const double*>(fp[c]);
-        gst[c] = fst[c];
-      }
-  }
-  // the cell pointers select between two buffers, which hides their address
-  // space from the compiler: the loads name global memory explicitly
-  using GD = const __attribute__((address_space(1))) double*;
-  using GS = const __attribute__((address_space(1))) S*;
-  // FP32 state: the FP64 ghost-strip load only in waves that hold a ghost cell
+  struct FPlaneW {
+    double g[NC];
+  };
+  using FPlane = std::conditional_t<kWide, FPlaneW, FPlaneN>;
+  // the ghost-strip load and the select only in waves that hold a ghost cell
   // (a wave-uniform branch; tiles off the panel edges have none)
   bool wave_g[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) wave_g[c] = kWide || __ballot(kind[c] == 2) != 0;
+  for (int c = 0; c < NC; ++c) wave_g[c] = __ballot(kind[c] == 2) != 0;
+  const size_t gstride = (size_t)3 * n;
+  // the loads name global memory explicitly (the pointers come from kernel
+  // arguments through a select the compiler cannot see through)
+  using GD = const __attribute__((address_space(1))) double*;
+  using GS = const __attribute__((address_space(1))) S*;
   auto load_plane = [&](int kk, FPlane& dst) {
+    const char* fb = reinterpret_cast<const char*>(g + (size_t)kk * plane);  // wave-uniform
+    const size_t goff = (size_t)kk * gstride;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if constexpr (kWide) {
-        dst.g[c] = ((GD)gp[c])[(unsigned)kk * gst[c]];
+        if (wave_g[c]) {
+          const double* p = kind[c] == 2 ? gp[c] + goff : reinterpret_cast<const double*>(fb + foff[c]);
+          dst.g[c] = *(GD)p;
+        } else {
+          dst.g[c] = *(GD)(fb + foff[c]);
+        }
       } else {
-        dst.f[c] = ((GS)fp[c])[(unsigned)kk * fst[c]];
-        dst.g[c] = wave_g[c] ? ((GD)gp[c])[(unsigned)kk * gst[c]] : 0.0;
+        dst.f[c] = *(GS)(fb + foff[c]);
+        dst.g[c] = wave_g[c] ? *(GD)(gp[c] + goff) : 0.0;
       }
     }
   };
   auto widen = [&](const FPlane& L, int c) -> double {
-    if constexpr (kWide) return kind[c] ? L.g[c] : 0.0;
-    const double fv = static_cast<double>(L.f[c]);
-    const double v = kind[c] == 1 ? fv : L.g[c];
-    return kind[c] ? v : 0.0;
+    if constexpr (kWide) {
+      return L.g[c];
+    } else {
+      const double fv = static_cast<double>(L.f[c]);
+      if (wave_g[c]) return kind[c] == 2 ? L.g[c] : fv;
+      return fv;
+    }
   };
 
   // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
   // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
+  // The sums start from the first product, not from the reference's 0.0: the
+  // two differ only in the sign of a zero result, and FILTER_ETA, whose sum
+  // starts from 0.0, returns the same bits for either sign of a zero input.
   // FP32 state: a task is two adjacent outputs of one row, read as four
   // aligned 16-byte pairs (eight values for two 7-tap sums), 22 x 16 tasks
   // for a 32 x 16 tile: half the LDS reads of one output per thread (filter
   // at c3 2.79 -> 2.61 ms).  The FP64 filter, bound by HBM at c3, runs one
   // output per thread: the pairs made it slower there (3.02 -> 3.15 ms).
+  typedef double dpair __attribute__((ext_vector_type(2)));  // one 16-byte LDS access
   auto xi_pass = [&]() {
-    if constexpr (kWide) {
+    if constexpr (sizeof(S) == sizeof(double)) {
       for (int rr = ty; rr < TJ + 6; rr += TJ) {
         const int gj = j0 - 3 + rr;
         double xv;
         if (gj >= 0 && gj < n) {
-          xv = 0.0;
+          xv = T.filt[0] * sq[rr][tx];
 #pragma unroll
-          for (int q = 0; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
+          for (int q = 1; q < 7; ++q) xv = xv + T.filt[q] * sq[rr][tx + q];
         } else {
           xv = sq[rr][tx + 3];
         }
@@ -926,21 +935,21 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     constexpr int NTASK = (TJ + 6) * (TI / 2);
     for (int t = tid; t < NTASK; t += NT) {
       const int rr = t / (TI / 2), c0 = 2 * (t % (TI / 2));
-      const double2* src = reinterpret_cast<const double2*>(&sq[rr][c0]);
+      const dpair* src = reinterpret_cast<const dpair*>(&sq[rr][c0]);
       double v[8];
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        const double2 d = src[h];
+        const dpair d = src[h];
         v[2 * h] = d.x;
         v[2 * h + 1] = d.y;
       }
       const int gj = j0 - 3 + rr;
       double x0, x1;
       if (gj >= 0 && gj < n) {
-        x0 = 0.0;
-        x1 = 0.0;
+        x0 = T.filt[0] * v[0];
+        x1 = T.filt[0] * v[1];
 #pragma unroll
-        for (int q = 0; q < 7; ++q) {
+        for (int q = 1; q < 7; ++q) {
           x0 = x0 + T.filt[q] * v[q];
           x1 = x1 + T.filt[q] * v[q + 1];
         }
@@ -948,7 +957,10 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
         x0 = v[3];
         x1 = v[4];
       }
-      *reinterpret_cast<double2*>(&sx[rr][c0]) = make_double2(x0, x1);
+      dpair out;
+      out.x = x0;
+      out.y = x1;
+      *reinterpret_cast<dpair*>(&sx[rr][c0]) = out;
     }
   };
 
@@ -963,16 +975,23 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     __syncthreads();
     xi_pass();
     __syncthreads();
-    double v = 0.0;
+    double v = 0.0;  // stored: the reference's sum from 0.0
 #pragma unroll
     for (int q = 0; q < 7; ++q) v = v + T.filt[q] * sx[ty + q][tx];
     if (own_in) o[ij] = static_cast<S>(v);
     return;
   }
 
-  double win[11];
+  // FILTER_R (filter.f90:682-693), output plane k = sum over q of
+  // filtr[q] * eta(k - 5 + q), summed from 0.0 in q order.  The taps are
+  // symmetric (filtr[q] == filtr[10 - q], checked when the tables are
+  // uploaded), so each arriving plane's six distinct products feed the eleven
+  // pending outputs: acc[d] is output plane kk - 5 + d, which takes the term
+  // q = 10 - d of plane kk, the same order and the same products as the
+  // sum over a window of the last eleven planes with 6 multiplies instead of 11.
+  double acc[11];
 #pragma unroll
-  for (int q = 0; q < 11; ++q) win[q] = 0.0;
+  for (int q = 0; q < 11; ++q) acc[q] = 0.0;
 
   const int kbeg = a - kNstr, kend = bnd + kNstr;  // kend - kbeg >= 11
   // store: the step completes an output plane (every step from the 11th on)
@@ -986,19 +1005,22 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     __syncthreads();
     xi_pass();
     __syncthreads();
-    // FILTER_ETA (filter.f90:448-629).
-    double v = 0.0;
+    // FILTER_ETA (filter.f90:448-629), from the first product (as FILTER_XI:
+    // FILTER_R's sum from 0.0 does not see the sign of a zero input).
+    double v = T.filt[0] * sx[ty][tx];
 #pragma unroll
-    for (int q = 0; q < 7; ++q) v = v + T.filt[q] * sx[ty + q][tx];
+    for (int q = 1; q < 7; ++q) v = v + T.filt[q] * sx[ty + q][tx];
+    double pr[6];
 #pragma unroll
-    for (int q = 0; q < 10; ++q) win[q] = win[q + 1];
-    win[10] = v;
-    if constexpr (decltype(store)::value) {  // FILTER_R interior (filter.f90:682-693)
-      double s = 0.0;
+    for (int m = 0; m < 6; ++m) pr[m] = T.filtr[m] * v;
 #pragma unroll
-      for (int q = 0; q < 11; ++q) s = s + T.filtr[q] * win[q];
-      if (own_in) o[(size_t)(kk - kNstr) * plane + ij] = static_cast<S>(s);
+    for (int d = 0; d < 10; ++d) acc[d] = acc[d] + pr[d < 6 ? d : 10 - d];
+    acc[10] = 0.0 + pr[0];
+    if constexpr (decltype(store)::value) {
+      if (own_in) o[(size_t)(kk - kNstr) * plane + ij] = static_cast<S>(acc[0]);
     }
+#pragma unroll
+    for (int d = 0; d < 10; ++d) acc[d] = acc[d + 1];
   };
   // Branch-free memory traffic per iteration (a fixed warm-up, then pairs of
   // storing steps, then at most one more), so the waits before each LDS
