@@ -306,14 +306,17 @@ __global__ __launch_bounds__(TI* TJ) void k_adj_stage(DevTables T, const double*
     }
     __syncthreads();
 
-    // Transposed stencils: (D^T u)(i) = sum_q c[6-q] u(i-3+q).
+    // Transposed stencils: (D^T u)(i) = sum_q c[6-q] u(i-3+q); one set of
+    // coefficients for xi and eta, whose zero centre tap adds nothing (as in
+    // the forward stage; both checked when the tables are uploaded).
     double dxv = 0.0, dxp = 0.0, dyv = 0.0, dyp = 0.0;
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
+      if (q == 3) continue;
       dxv += T.fcx[6 - q] * sxv[bf][ty][tx + q];
       dxp += T.fcx[6 - q] * sxp[bf][ty][tx + q];
-      dyv += T.fcy[6 - q] * syv[bf][ty + q][tx];
-      dyp += T.fcy[6 - q] * syp[bf][ty + q][tx];
+      dyv += T.fcx[6 - q] * syv[bf][ty + q][tx];
+      dyp += T.fcx[6 - q] * syp[bf][ty + q][tx];
     }
     // Transposed radial stencil: every row kk whose DDR stencil reads plane k.
     double drv = 0.0, drp = 0.0;

@@ -291,6 +291,11 @@ int solar::upload_tables(const Tables& t, DevTables* d, std::vector<void*>* allo
     d->filt[q] = solar::kFiltCoeffs[q];
   }
   for (int q = 0; q < 11; ++q) d->filtr[q] = solar::kFiltCoeffsR[q];
+  // the stage kernel uses fcx for both angular stencils and skips their zero
+  // centre tap
+  for (int q = 0; q < 7; ++q)
+    if (d->fcx[q] != d->fcy[q]) return fail(SOLAR_ERR_ARG, "xi and eta stencil coefficients differ");
+  if (d->fcx[3] != 0.0) return fail(SOLAR_ERR_ARG, "the angular stencil's centre tap is not zero");
   // the filter kernel computes each radial product once for taps q and 10 - q
   for (int q = 0; q < 11; ++q)
     if (d->filtr[q] != d->filtr[10 - q]) return fail(SOLAR_ERR_ARG, "radial filter taps are not symmetric");

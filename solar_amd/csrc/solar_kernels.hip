@@ -676,7 +676,11 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     __syncthreads();
 
     // 7-point DRP stencils, ascending taps from 0 (derivative.f90:137-148,
-    // 456-467, 656-668, 704-754).
+    // 456-467, 656-668, 704-754).  In xi and eta the coefficients are the
+    // same (d_eta = d_xi) and the centre tap is 0 (both checked when the
+    // tables are uploaded): its product, +-0 for a finite value, leaves the
+    // sum unchanged, so it is not formed.  Fewer uniform operands also keep
+    // the kernel's scalar registers from spilling.
     double dvx = 0.0, dve = 0.0, dpx = 0.0, dpe = 0.0, dvr = 0.0, dpr = 0.0;
     // the radial row is block-uniform and read-only: through the constant
     // address space it comes in with scalar loads, which the vector-load
@@ -684,10 +688,12 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     const auto& row = const_rows(T.rad)[k];
 #pragma unroll
     for (int q = 0; q < 7; ++q) {
-      dvx = dvx + T.fcx[q] * svx[bf][ty][tx + q];
-      dve = dve + T.fcy[q] * sve[bf][ty + q][tx];
-      dpx = dpx + T.fcx[q] * sp[bf][ty + 3][tx + q];
-      dpe = dpe + T.fcy[q] * sp[bf][ty + q][tx + 3];
+      if (q != 3) {
+        dvx = dvx + T.fcx[q] * svx[bf][ty][tx + q];
+        dve = dve + T.fcx[q] * sve[bf][ty + q][tx];
+        dpx = dpx + T.fcx[q] * sp[bf][ty + 3][tx + q];
+        dpe = dpe + T.fcx[q] * sp[bf][ty + q][tx + 3];
+      }
       dvr = dvr + row.w[q] * rw[q];
       dpr = dpr + row.w[q] * pw[q];
     }
