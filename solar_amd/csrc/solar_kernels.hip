@@ -40,18 +40,52 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? l
 // negated remainder keeps the sign of a zero quotient (-0 / b = -0).  The
 // remainders and quotients must stay normal: with the divisors in
 // [2^-60, 2^60] (fast, checked on the host) a numerator whose exponent lies
-// in [-900, 900], or zero, takes this path; others (subnormal field values
-// occur in the tails of Gaussian pulses) take the hardware division, as do
-// all when fast is false.  An infinite numerator gives NaN here where the
-// division gives an infinity (both non-finite).  Five FP64 operations and an
-// exponent test, against about a dozen with a quarter-rate reciprocal.
-__device__ __forceinline__ double div_by(double a, double b, double y, bool fast) {
-  if (__builtin_expect(!fast || (unsigned)(__builtin_amdgcn_frexp_exp(a) + 900) > 1800u, 0)) return a / b;
+// in [-900, 900], or zero, may take this path; others (subnormal field
+// values occur in the tails of Gaussian pulses) take the hardware division
+// (div_group), as do all when fast is false.  An infinite numerator gives
+// NaN here where the division gives an infinity (both non-finite).  Five
+// FP64 operations and an exponent test, against about a dozen with a
+// quarter-rate reciprocal.
+__device__ __forceinline__ double quot_rcp(double a, double b, double y) {
   const double q0 = a * y;
   const double r0 = __builtin_fma(b, q0, -a);
   const double q1 = __builtin_fma(-r0, y, q0);
   const double r1 = __builtin_fma(b, q1, -a);
   return __builtin_fma(-r1, y, q1);
+}
+// The numerator test: above 1800 when the exponent lies outside [-900, 900].
+__device__ __forceinline__ unsigned quot_exp(double a) { return (unsigned)(__builtin_amdgcn_frexp_exp(a) + 900); }
+// A group of quotients with one test (RCP): every lane takes the reciprocal
+// sequence; in a wave where some lane's numerators fail the test (or fast
+// is false) those lanes redo the group by hardware division.  The common
+// path then has no branch per quotient.  Without RCP, hardware division.
+// Measured in one call: the FP32-state step at c3 16.23 -> 16.03 ms against
+// a guard per quotient; the FP64 stages, latency-bound at two waves per
+// SIMD, were no faster at c3 and slower at 6x128^2x256 and 6x96^2x192
+// (+0.5%, +1.6%) than with hardware division, and keep it (a guard per
+// quotient had been slower there too: it drained the prefetch at the
+// loop's back edge).
+template <bool RCP, int N>
+__device__ __forceinline__ void div_group(const double (&a)[N], const double (&b)[N], const double (&y)[N], bool fast,
+                                          double (&q)[N]) {
+  if constexpr (!RCP) {
+#pragma unroll
+    for (int t = 0; t < N; ++t) q[t] = a[t] / b[t];
+    return;
+  }
+  unsigned m = 0;
+#pragma unroll
+  for (int t = 0; t < N; ++t) {
+    q[t] = quot_rcp(a[t], b[t], y[t]);
+    m = max(m, quot_exp(a[t]));
+  }
+  const bool slow = !fast || m > 1800u;
+  if (__builtin_expect(__ballot(slow) != 0, 0)) {
+    if (slow) {
+#pragma unroll
+      for (int t = 0; t < N; ++t) q[t] = a[t] / b[t];
+    }
+  }
 }
 
 // Source point (i,j) of along-edge index a on the strip of edge e at layer l
@@ -550,11 +584,10 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
 
   const double Ci = to_global(T.ci)[ic], Dj = to_global(T.dj)[jc];
   const double yCi = 1.0 / Ci, yDj = 1.0 / Dj;  // the divisors fixed for this thread's column
-  // Reciprocal-based quotients in the FP32-state stages only: c3 stage 2.87 ->
-  // 2.74 ms.  In the FP64 stages the rare-path branches made the compiler
-  // drain the prefetch at the loop's back edge (vmcnt(0)), and the stages got
-  // slower (4.48 -> 4.51 ms at c3); they divide in hardware.
-  const bool fdiv = sizeof(S) == 4 && T.fast_div != 0;
+  // Reciprocal-based quotients in the FP32-state stages only (c3 stage 2.87 ->
+  // 2.74 ms with a guard per quotient, and a further 1% with one guard per
+  // group); the FP64 stages divide in hardware (div_group).
+  const bool fdiv = T.fast_div != 0;
   const double dc = to_global(T.dc)[ij], tdx = to_global(T.tdx)[ij], tde = to_global(T.tde)[ij],
                xy = to_global(T.xy)[ij];
 
@@ -699,18 +732,34 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
     }
 
     // grad.f90:125-133 and :49-57.
-    const double div = (row.inv_r * ((dc * (div_by(dvx, Ci, yCi, fdiv) + div_by(dve, Dj, yDj, fdiv)) - tdx * vx) - tde * ve) +
-                        dvr) + div_by(2.0 * rc, row.r, row.inv_r, fdiv);
-    const double gxi = row.inv_r * (Dj * dpx + div_by(xy * dpe, Dj, yDj, fdiv));
-    const double get = row.inv_r * (Ci * dpe + div_by(xy * dpx, Ci, yCi, fdiv));
+    // quotients by C, D, r and rho0 (grad.f90:52-53, 128-130;
+    // compute_rhs_acoustic.f90:88-146), in two groups: the second divides
+    // the first's results
+    double qa[7];
+    {
+      const double na[7] = {dvx, dve, 2.0 * rc, xy * dpe, xy * dpx, dpr, q1 * row.g0};
+      const double ba[7] = {Ci, Dj, row.r, Dj, Ci, row.rho0, row.rho0};
+      const double ya[7] = {yCi, yDj, row.inv_r, yDj, yCi, row.inv_rho0, row.inv_rho0};
+      div_group<kWide ? false : true>(na, ba, ya, fdiv, qa);
+    }
+    const double div = (row.inv_r * ((dc * (qa[0] + qa[1]) - tdx * vx) - tde * ve) + dvr) + qa[2];
+    const double gxi = row.inv_r * (Dj * dpx + qa[3]);
+    const double get = row.inv_r * (Ci * dpe + qa[4]);
 
     // compute_rhs_acoustic.f90:38-149 (forcing omitted).
     double r1 = -(row.rho0 * div);
     r1 = r1 - rc * row.drho0dr;
-    double r2 = -div_by(gxi, row.rho0, row.inv_rho0, fdiv);
-    double r3 = -div_by(get, row.rho0, row.inv_rho0, fdiv);
-    double r4 = -div_by(dpr, row.rho0, row.inv_rho0, fdiv);
-    r4 = r4 - div_by(q1 * row.g0, row.rho0, row.inv_rho0, fdiv);
+    double qb[2];
+    {
+      const double nb[2] = {gxi, get};
+      const double bb[2] = {row.rho0, row.rho0};
+      const double yb[2] = {row.inv_rho0, row.inv_rho0};
+      div_group<kWide ? false : true>(nb, bb, yb, fdiv, qb);
+    }
+    double r2 = -qb[0];
+    double r3 = -qb[1];
+    double r4 = -qa[5];
+    r4 = r4 - qa[6];
     // c2f: optional 3-D c^2 (n,n,nr,6); the radial profile otherwise
     const double c2rho0 = c2f ? cur.c2 * row.rho0 : row.c2rho0;
     const size_t dx = (size_t)P * nf + (size_t)k * plane + ij;
@@ -772,9 +821,9 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
 //   fgx/fgy: ghosts of the unfiltered field (filter.f90:25-28), layouts as in
 //   k_gather, 12 sides (6 panels x 2) per field.
 // Per plane: the (TJ+6) x (TI+6) tile of the unfiltered field goes to LDS
-// (prefetched into registers one plane ahead), FILTER_XI runs on the in-panel
-// rows, FILTER_ETA on the result, and FILTER_R from an 11-plane register
-// window.
+// (prefetched into registers two planes ahead), FILTER_XI runs on the
+// in-panel rows, FILTER_ETA on the result, and FILTER_R accumulates the
+// plane's terms into the eleven pending outputs in registers.
 template <int TI, int TJ>
 struct FilterLds {
   alignas(16) double sq[TJ + 6][TI + 6];  // rows of TI + 6 doubles: 16-byte aligned pairs

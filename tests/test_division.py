@@ -1,8 +1,9 @@
-"""The FP32-state stage divides by C, D, r and rho0 through their reciprocals
-(div_by in solar_amd/csrc/solar_kernels.hip): q = RN(a y) with y = RN(1/b),
-then two corrections q <- RN(q - r y) with the negated remainder r = b q - a
-(FMA), numerators with a frexp exponent outside [-900, 900] taken by the
-hardware division, and the divisors held to [2^-60, 2^60] by the host.  These
+"""The stage kernels divide by C, D, r and rho0 through their reciprocals
+(quot_rcp and div_group in solar_amd/csrc/solar_kernels.hip): q = RN(a y)
+with y = RN(1/b), then two corrections q <- RN(q - r y) with the negated
+remainder r = b q - a (FMA), numerators with a frexp exponent outside
+[-900, 900] taken by the hardware division, and the divisors held to
+[2^-60, 2^60] by the host.  These
 CPU tests restate that sequence with an exact FMA (rational arithmetic,
 rounded once) and check it against IEEE division, bit for bit, on the
 divisors of real grids and on numerators across the whole double range:
