@@ -963,8 +963,11 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   // FILTER_XI on in-panel rows (filter.f90:176-315); rows beyond the panel
   // edge keep the unfiltered ghost, which FILTER_ETA uses (filter.f90:43).
   // The sums start from the first product, not from the reference's 0.0: the
-  // two differ only in the sign of a zero result, and FILTER_ETA, whose sum
-  // starts from 0.0, returns the same bits for either sign of a zero input.
+  // two differ only in the sign of a zero result.  A sum's value does not
+  // depend on the signs of zero terms, so FILTER_ETA's outputs keep the
+  // reference's values (possibly with another zero sign), and the stored
+  // sum -- FILTER_R's, or FILTER_ETA's in 2-D, both from 0.0, which never
+  // gives -0 -- has the reference's bits.
   // FP32 state: a task is two adjacent outputs of one row, read as four
   // aligned 16-byte pairs (eight values for two 7-tap sums), 22 x 16 tasks
   // for a 32 x 16 tile: half the LDS reads of one output per thread (filter
@@ -1060,8 +1063,8 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
     __syncthreads();
     xi_pass();
     __syncthreads();
-    // FILTER_ETA (filter.f90:448-629), from the first product (as FILTER_XI:
-    // FILTER_R's sum from 0.0 does not see the sign of a zero input).
+    // FILTER_ETA (filter.f90:448-629), from the first product, as FILTER_XI:
+    // FILTER_R's sum from 0.0 does not see the sign of a zero input.
     double v = T.filt[0] * sx[ty][tx];
 #pragma unroll
     for (int q = 1; q < 7; ++q) v = v + T.filt[q] * sx[ty + q][tx];
