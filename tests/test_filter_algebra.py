@@ -7,8 +7,9 @@ share floating-point operations whose results cannot change a stored bit:
     products per output over a window of the last eleven planes;
   - FILTER_XI and FILTER_ETA sum from their first product instead of from
     0.0 (filter.f90:176-182, 448-455): that changes at most the sign of a
-    zero result, and the next pass, a sum from 0.0, gives the same bits for
-    either sign of a zero input;
+    zero result, a sum's value does not depend on the signs of zero terms,
+    and the stored sum (FILTER_R's, from 0.0) never gives -0, so the output
+    bits are the reference's;
   - the stage's xi/eta DRP stencils (derivative.f90:137-148, 456-467) form
     no product for the zero centre tap.
 
@@ -128,3 +129,30 @@ def test_zero_centre_tap_leaves_stencil_sums_unchanged():
     # a non-finite centre value is where the two differ (0 * inf = nan)
     u = [1.0, 2.0, 3.0, math.inf, 5.0, 6.0, 7.0]
     assert math.isnan(_sum_from_zero(DRP, u))
+
+
+def test_filter_chain_equals_reference_bits():
+    """FILTER_XI, FILTER_ETA and FILTER_R on 11 planes of 7 x 7 values: the
+    kernel's forms (first-product xi and eta, shared radial products) against
+    the reference's (every sum from 0.0, a window per output)."""
+    rng = random.Random(21)
+    for trial in range(300):
+        if trial % 5 == 0:  # blocks of signed zeros whose products are all -0 in some rows
+            u = [[[(-0.0 if FILT[i] > 0 else 0.0) if rng.random() < 0.8 else rng.choice([1e-310, -2.0, 0.5])
+                   for i in range(7)] for _ in range(7)] for _ in range(11)]
+        else:
+            u = [[_adversarial(rng, 7) for _ in range(7)] for _ in range(11)]
+        x_ref = [[_sum_from_zero(FILT, row) for row in plane] for plane in u]
+        e_ref = [_sum_from_zero(FILT, xs) for xs in x_ref]
+        out_ref = _sum_from_zero(FILTR, e_ref)
+        x_k = [[_sum_from_first(FILT, row) for row in plane] for plane in u]
+        e_k = [_sum_from_first(FILT, xs) for xs in x_k]
+        acc = [0.0] * 11
+        for p, v in enumerate(e_k):
+            pr = [FILTR[m] * v for m in range(6)]
+            for d in range(10):
+                acc[d] = acc[d] + pr[d if d < 6 else 10 - d]
+            acc[10] = 0.0 + pr[0]
+            out_k = acc[0]
+            acc = acc[1:] + [acc[10]]
+        assert _bits(out_k) == _bits(out_ref), trial
