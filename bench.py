@@ -9,7 +9,9 @@ explicit filter over the whole six-panel grid, state resident in HBM.
 value = 6*n*n*nr*K / (wall seconds of K steps), grid-point RK4 steps/s.
 
 The headline workload is the 6x256x256x384 FP64 grid (c3); the line also
-carries the 6x128x128x256 grid (c2) as a nested object.
+carries the 6x128x128x256 grid (c2) as a nested object, and (fp32_state) the
+same two grids with an FP32 state: step, stage and filter times and the step
+speed-up over the FP64 state.
 
 Printed as one JSON line, with
   roofline     : the dominant kernel (the fused stage kernel), algorithmic
@@ -268,6 +270,7 @@ def main():
                     help="storage of the state and 2N accumulator (arithmetic is FP64 either way)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint cost measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-state measurement of the FP64 line")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference gradient check")
     ap.add_argument("--adjoint-steps", type=int, default=None, help="steps of the adjoint workload (default 2000)")
@@ -364,6 +367,14 @@ def main():
     if not args.no_c3 and args.second not in ("none", args.workload):
         second = _grid_bench(args.second, 200, 0 if args.no_cpu or bits != 64 else 1, bits)
 
+    fp32 = None
+    if not args.no_fp32 and bits == 64:
+        fp32 = {cfg.name: _grid_bench(cfg.name, 200, 0, 32)}
+        fp32[cfg.name]["over_fp64"] = (wall / steps * 1e3) / fp32[cfg.name]["ms_per_step"]
+        if second:
+            fp32[args.second] = _grid_bench(args.second, 200, 0, 32)
+            fp32[args.second]["over_fp64"] = second["ms_per_step"] / fp32[args.second]["ms_per_step"]
+
     ensemble = None
     if not args.no_ensemble and bits == 64:
         ensemble = _ensemble_bench(args.ensemble_steps, 10)
@@ -389,6 +400,8 @@ def main():
         out["adjoint"] = adjoint
     if ensemble:
         out["ensemble"] = ensemble
+    if fp32:
+        out["fp32_state"] = fp32
     if second:
         out[args.second] = second
     if cpu:
