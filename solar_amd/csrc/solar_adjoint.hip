@@ -30,6 +30,15 @@
 #include "solar_adjoint.hpp"
 #include "solar_device.hpp"
 
+// The library is built with -ffp-contract=off, so that the forward kernels
+// round every product and sum as the reference does.  The adjoint has no
+// reference bits to match: it is checked by transpose identities (to 1e-13
+// of |Ax||y|), the tangent-linear dot product and finite differences.  Its
+// kernels fuse products into sums (one rounding per multiply-add), which is
+// as deterministic as the separate form: the gradient still repeats bit for
+// bit across runs, checkpoint spacings and cache settings.
+#pragma clang fp contract(fast)
+
 namespace solar {
 
 namespace {
