@@ -200,7 +200,7 @@ def _grid_bench(name, steps, cpu_steps, bits=64):
     os.environ.pop("SOLAR_PROFILE")
     p.set_state(q0)
     p.rk4(2, dt)
-    p.rk4(5, dt)
+    p.rk4(20, dt)
     kt = p.kernel_times()
     ab = p.algorithmic_bytes()
     p.close()
