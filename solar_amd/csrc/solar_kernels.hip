@@ -833,7 +833,8 @@ struct FilterLds {
 // The tile (bi, bj) of field F of panel P (PF = 5P + F) over planes [k0, k1).
 // PART: the tile may reach past the panel edge; threads past it store nothing,
 // and tile cells beyond the panel's three ghost layers stay 0.
-template <int TI, int TJ, typename S, bool PART = false>
+// PAIR: FILTER_XI tasks of two adjacent outputs (below).
+template <int TI, int TJ, typename S, bool PART = false, bool PAIR = (sizeof(S) == 4)>
 __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, int PF, int k0, int k1,
                                             const S* __restrict__ qin, S* __restrict__ qout,
                                             const double* __restrict__ fgx, const double* __restrict__ fgy,
@@ -968,14 +969,16 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
   // reference's values (possibly with another zero sign), and the stored
   // sum -- FILTER_R's, or FILTER_ETA's in 2-D, both from 0.0, which never
   // gives -0 -- has the reference's bits.
-  // FP32 state: a task is two adjacent outputs of one row, read as four
-  // aligned 16-byte pairs (eight values for two 7-tap sums), 22 x 16 tasks
-  // for a 32 x 16 tile: half the LDS reads of one output per thread (filter
-  // at c3 2.79 -> 2.61 ms).  The FP64 filter, bound by HBM at c3, runs one
-  // output per thread: the pairs made it slower there (3.02 -> 3.15 ms).
+  // PAIR: a task is two adjacent outputs of one row, read as four aligned
+  // 16-byte pairs (eight values for two 7-tap sums), 22 x 16 tasks for a
+  // 32 x 16 tile: half the LDS reads of one output per thread (FP32 filter
+  // at c3 2.79 -> 2.61 ms).  The FP64 filter takes pairs on panels up to
+  // 128 (filter 6x128^2x256 -3%, 6x96^2x192 -7%, the ensemble step -0.8%)
+  // and one output per thread on larger ones (6x256^2x384: 2.93 -> 3.14 ms
+  // with pairs).
   typedef double dpair __attribute__((ext_vector_type(2)));  // one 16-byte LDS access
   auto xi_pass = [&]() {
-    if constexpr (sizeof(S) == sizeof(double)) {
+    if constexpr (!PAIR) {
       for (int rr = ty; rr < TJ + 6; rr += TJ) {
         const int gj = j0 - 3 + rr;
         double xv;
@@ -1108,7 +1111,7 @@ __device__ __forceinline__ void filter_tile(const DevTables& T, int bi, int bj, 
 template <int TI, int TJ, typename S>
 constexpr int filter_waves() { return (sizeof(S) == 4 && TI * TJ >= 256 && !(TI == 64 && TJ == 4) && !(TI == 32 && TJ == 8)) ? 6 : 1; }
 
-template <int TI, int TJ, typename S, bool PART = false>
+template <int TI, int TJ, typename S, bool PART = false, bool PAIR = (sizeof(S) == 4)>
 __global__ __launch_bounds__(TI* TJ) __attribute__((amdgpu_waves_per_eu(filter_waves<TI, TJ, S>())))
 void k_filter(DevTables T, const S* __restrict__ qin,
                                                     S* __restrict__ qout, const double* __restrict__ fgx,
@@ -1127,12 +1130,13 @@ void k_filter(DevTables T, const S* __restrict__ qin,
   const int k0 = ch * kchunk;
   const int k1 = min(k0 + kchunk, nr);
   if (k0 >= nr) return;
-  filter_tile<TI, TJ, S, PART>(T, bi, bj, PF, k0, k1, qin, qout, fgx, fgy, lds);
+  filter_tile<TI, TJ, S, PART, PAIR>(T, bi, bj, PF, k0, k1, qin, qout, fgx, fgy, lds);
 }
 
 // Ensemble FILTER_EXP: work items (member, tile, 5P+F, plane range), q[cur] -> q[1-cur].
-// Six waves per SIMD (80 registers, a 20-byte spill at 32x16): the 48-member
-// step 37.47 -> 37.01 ms against five waves at 88 registers.
+// FILTER_XI in paired tasks (the members are small panels).  Six waves per
+// SIMD (80 registers, a 32-byte spill at 32x16): faster than five waves
+// without the spill (37.22 against 37.64 ms per 48-member step).
 template <int TI, int TJ>
 __global__ __launch_bounds__(TI* TJ) __attribute__((amdgpu_waves_per_eu(6))) void k_filter_ens(const EnsMember* __restrict__ M, const int4* __restrict__ items,
                                                         int nitems, int cur) {
@@ -1140,7 +1144,7 @@ __global__ __launch_bounds__(TI* TJ) __attribute__((amdgpu_waves_per_eu(6))) voi
   for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
     const int4 w = items[item];
     const EnsMember& E = M[w.x];
-    filter_tile<TI, TJ, double>(E.T, w.y & 0xffff, w.y >> 16, w.z & 0xff, w.z >> 8, w.w, E.q[cur], E.q[1 - cur],
+    filter_tile<TI, TJ, double, false, true>(E.T, w.y & 0xffff, w.y >> 16, w.z & 0xff, w.z >> 8, w.w, E.q[cur], E.q[1 - cur],
                                 E.fgx, E.fgy, lds);
     __syncthreads();  // the next item rewrites the LDS tiles
   }
@@ -1386,7 +1390,7 @@ hipError_t launch_stage(const DevTables& T, const void* qin, void* qout, void* q
                         static_cast<double*>(ecols), xf, static_cast<double*>(qwo), beta_r);
 }
 
-template <int TI, int TJ, typename S, bool PART = false>
+template <int TI, int TJ, typename S, bool PART = false, bool PAIR = (sizeof(S) == 4)>
 static hipError_t launch_filter_t(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
                                   hipStream_t st, const LaunchShape& ls) {
   const int tiles = ((T.n + TI - 1) / TI) * ((T.n + TJ - 1) / TJ) * 30;
@@ -1395,21 +1399,29 @@ static hipError_t launch_filter_t(const DevTables& T, const S* qin, S* qout, con
   const int nkch = ls.filter_chunks > 0 ? ls.filter_chunks : chunks_for(tiles, T.nr, 4096, 64);
   const int kchunk = (T.nr + nkch - 1) / nkch;
   dim3 grid(tiles * nkch), block(TI, TJ);
-  hipLaunchKernelGGL((k_filter<TI, TJ, S, PART>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
+  hipLaunchKernelGGL((k_filter<TI, TJ, S, PART, PAIR>), grid, block, 0, st, T, qin, qout, fgx, fgy, kchunk, nkch);
   return hipGetLastError();
 }
 
 template <typename S>
 static hipError_t launch_filter_s(const DevTables& T, const S* qin, S* qout, const double* fgx, const double* fgy,
                                   hipStream_t st, const LaunchShape& ls) {
-  if (T.n % 16 != 0)  // partial tiles
-    return pad_pick_ti(T.n) == 32 ? launch_filter_t<32, 16, S, true>(T, qin, qout, fgx, fgy, st, ls)
-                                  : launch_filter_t<16, 16, S, true>(T, qin, qout, fgx, fgy, st, ls);
+  // FILTER_XI in paired tasks: always for an FP32 state, for FP64 on panels up to 128
+  const bool pair = sizeof(S) == 4 || T.n <= 128;
+  if (T.n % 16 != 0) {  // partial tiles
+    if (pad_pick_ti(T.n) == 32)
+      return pair ? launch_filter_t<32, 16, S, true, true>(T, qin, qout, fgx, fgy, st, ls)
+                  : launch_filter_t<32, 16, S, true, false>(T, qin, qout, fgx, fgy, st, ls);
+    return pair ? launch_filter_t<16, 16, S, true, true>(T, qin, qout, fgx, fgy, st, ls)
+                : launch_filter_t<16, 16, S, true, false>(T, qin, qout, fgx, fgy, st, ls);
+  }
   int ti = ls.filter_ti, tj = ls.filter_tj;
   if (!fits(T.n, ti, tj)) {  // measured best: 32x16 (n%32==0), 16x16
     ti = T.n % 32 == 0 ? 32 : 16;
     tj = 16;
   }
+  if (pair && ti == 32 && tj == 16) return launch_filter_t<32, 16, S, false, true>(T, qin, qout, fgx, fgy, st, ls);
+  if (pair && ti == 16 && tj == 16) return launch_filter_t<16, 16, S, false, true>(T, qin, qout, fgx, fgy, st, ls);
 #define SOLAR_FILTER_CASE(A, B) \
   if (ti == A && tj == B) return launch_filter_t<A, B, S>(T, qin, qout, fgx, fgy, st, ls);
   SOLAR_FILTER_CASE(32, 16)
@@ -1419,8 +1431,11 @@ static hipError_t launch_filter_s(const DevTables& T, const S* qin, S* qout, con
   SOLAR_FILTER_CASE(64, 8)
   SOLAR_FILTER_CASE(64, 16)
 #undef SOLAR_FILTER_CASE
-  if (T.n % 32 == 0) return launch_filter_t<32, 16, S>(T, qin, qout, fgx, fgy, st, ls);
-  return launch_filter_t<16, 16, S>(T, qin, qout, fgx, fgy, st, ls);
+  if (T.n % 32 == 0)
+    return pair ? launch_filter_t<32, 16, S, false, true>(T, qin, qout, fgx, fgy, st, ls)
+                : launch_filter_t<32, 16, S>(T, qin, qout, fgx, fgy, st, ls);
+  return pair ? launch_filter_t<16, 16, S, false, true>(T, qin, qout, fgx, fgy, st, ls)
+              : launch_filter_t<16, 16, S>(T, qin, qout, fgx, fgy, st, ls);
 }
 
 hipError_t launch_filter(const DevTables& T, const void* qin, void* qout, bool f32, const double* fgx,
