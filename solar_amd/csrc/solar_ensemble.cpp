@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -51,9 +52,16 @@ struct solar_ens {
   EnsMember* dmem = nullptr;
   std::vector<Choice> choice;
   std::vector<Launch> stage_l, filter_l;
-  int* gprefix = nullptr;
-  int gblocks = 0, maxn = 0;
-  int persist = 0;  // > 0: grid cap of the work-list launches
+  // gathers: one launch per panel size n (its block size is 3n strip
+  // points); table = first block of each member of the group, then the
+  // members' indices
+  struct GatherGroup {
+    int* table = nullptr;
+    int nm = 0, nblocks = 0, n = 0;
+  };
+  std::vector<GatherGroup> gg;
+  int gchunks = 4;  // radial chunks per receiver edge in the gathers
+  int persist = 0;   // > 0: grid cap of the work-list launches
   std::vector<void*> allocs;
   std::vector<void*> list_allocs;  // work lists, rebuilt by the autotuner
   size_t bytes = 0;
@@ -169,15 +177,23 @@ int build_lists(solar_ens* e) {
   return SOLAR_OK;
 }
 
+// Every stage also writes its output's W/E edge columns to the member's
+// ecols, from which the next gather reads its column strips (compact lines
+// instead of three values from every row of the state); the step's first
+// gather reads the filtered state.
 int enqueue_step(solar_ens* e) {
   int cur = 0;
   for (int s = 0; s < 5; ++s) {
-    ENSCHK(solar::launch_gather_ens(e->dmem, e->gprefix, e->nm, e->gblocks, e->maxn, true, cur, e->stream));
+    for (const auto& G : e->gg)
+      ENSCHK(solar::launch_gather_ens(e->dmem, G.table, G.nm, G.nblocks, G.n, true, cur, s > 0, e->gchunks,
+                                      e->stream));
     for (const Launch& L : e->stage_l)
       ENSCHK(solar::launch_stage_ens(e->dmem, L.items, L.nitems, L.grid, L.ti, L.tj, s, cur, e->stream));
     cur ^= 1;
   }
-  ENSCHK(solar::launch_gather_ens(e->dmem, e->gprefix, e->nm, e->gblocks, e->maxn, false, cur, e->stream));
+  for (const auto& G : e->gg)
+    ENSCHK(solar::launch_gather_ens(e->dmem, G.table, G.nm, G.nblocks, G.n, false, cur, true, e->gchunks,
+                                    e->stream));
   for (const Launch& L : e->filter_l)
     ENSCHK(solar::launch_filter_ens(e->dmem, L.items, L.nitems, L.grid, L.ti, L.tj, cur, e->stream));
   return SOLAR_OK;
@@ -238,9 +254,12 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
   e->nr.assign(nr, nr + nmembers);
   e->mem.resize(nmembers);
   e->choice.resize(nmembers);
+  if (const char* v = std::getenv("SOLAR_ENS_GATHER_CHUNKS")) {  // launch-shape sweeps
+    const int c = std::atoi(v);
+    if (c >= 1 && c <= 64) e->gchunks = c;
+  }
   if (hipStreamCreate(&e->stream) != hipSuccess) return bail(solar::set_error(SOLAR_ERR_HIP, "hipStreamCreate failed"));
   size_t roff = 0;
-  std::vector<int> prefix(nmembers);
   for (int m = 0; m < nmembers; ++m) {
     if (!(dt[m] > 0.0))
       return bail(solar::set_error(SOLAR_ERR_ARG, "solar_ens_create: member " + std::to_string(m) + ": dt <= 0"));
@@ -261,9 +280,9 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
     if (rc) return bail(rc);
     const size_t nq = static_cast<size_t>(n[m]) * n[m] * nr[m] * 30;
     const size_t ng = static_cast<size_t>(12) * nr[m] * n[m] * 3;
-    void* p[9];
-    const size_t sz[9] = {nq, nq, nq, ng, ng, ng, ng, 5 * ng, 5 * ng};
-    for (int a = 0; a < 9; ++a) {
+    void* p[10];
+    const size_t sz[10] = {nq, nq, nq, ng, ng, ng, ng, 5 * ng, 5 * ng, solar::edge_cols_elems(n[m], nr[m])};
+    for (int a = 0; a < 10; ++a) {
       if ((rc = ens_alloc(e, &p[a], sz[a] * sizeof(double), &e->allocs))) return bail(rc);
       if (hipMemset(p[a], 0, sz[a] * sizeof(double)) != hipSuccess)
         return bail(solar::set_error(SOLAR_ERR_HIP, "solar_ens_create: hipMemset failed"));
@@ -277,10 +296,8 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
     M.gyv = static_cast<double*>(p[6]);
     M.fgx = static_cast<double*>(p[7]);
     M.fgy = static_cast<double*>(p[8]);
+    M.ecols = static_cast<double*>(p[9]);
     M.dt = dt[m];
-    prefix[m] = e->gblocks;
-    e->gblocks += 24 * solar::gather_blocks_per_edge(nr[m]);
-    e->maxn = std::max(e->maxn, n[m]);
     e->choice[m] = default_choice(n[m], nr[m]);
   }
   void* d = nullptr;
@@ -289,10 +306,25 @@ int solar_ens_create(solar_ens** out, int nmembers, const int* n, const int* nr,
   e->dmem = static_cast<EnsMember*>(d);
   if (hipMemcpy(e->dmem, e->mem.data(), nmembers * sizeof(EnsMember), hipMemcpyHostToDevice) != hipSuccess)
     return bail(solar::set_error(SOLAR_ERR_HIP, "solar_ens_create: member table upload failed"));
-  if ((rc = ens_alloc(e, &d, nmembers * sizeof(int), &e->allocs))) return bail(rc);
-  e->gprefix = static_cast<int*>(d);
-  if (hipMemcpy(e->gprefix, prefix.data(), nmembers * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
-    return bail(solar::set_error(SOLAR_ERR_HIP, "solar_ens_create: prefix upload failed"));
+  std::map<int, std::vector<int>> by_n;
+  for (int m = 0; m < nmembers; ++m) by_n[n[m]].push_back(m);
+  for (auto& kv : by_n) {
+    solar_ens::GatherGroup G;
+    G.n = kv.first;
+    G.nm = static_cast<int>(kv.second.size());
+    std::vector<int> table(2 * G.nm);
+    for (int x = 0; x < G.nm; ++x) {
+      const int m = kv.second[x];
+      table[x] = G.nblocks;
+      table[G.nm + x] = m;
+      G.nblocks += 24 * solar::gather_blocks_per_edge(nr[m], e->gchunks);
+    }
+    if ((rc = ens_alloc(e, &d, table.size() * sizeof(int), &e->allocs))) return bail(rc);
+    G.table = static_cast<int*>(d);
+    if (hipMemcpy(G.table, table.data(), table.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(solar::set_error(SOLAR_ERR_HIP, "solar_ens_create: gather table upload failed"));
+    e->gg.push_back(G);
+  }
   if ((rc = build_lists(e))) return bail(rc);
   if ((rc = clear_state(e))) return bail(rc);
   *out = e;
@@ -457,7 +489,8 @@ int solar_ens_info(solar_ens* e, size_t* points, int* launches_per_step, size_t*
   for (int m = 0; m < e->nm; ++m) p += static_cast<size_t>(6) * e->n[m] * e->n[m] * e->nr[m];
   if (points) *points = p;
   if (launches_per_step)
-    *launches_per_step = 5 * (1 + static_cast<int>(e->stage_l.size())) + 1 + static_cast<int>(e->filter_l.size());
+    *launches_per_step = 6 * static_cast<int>(e->gg.size()) + 5 * static_cast<int>(e->stage_l.size()) +
+                         static_cast<int>(e->filter_l.size());
   if (device_bytes) *device_bytes = e->bytes;
   return SOLAR_OK;
 }

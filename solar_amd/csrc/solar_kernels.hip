@@ -343,10 +343,11 @@ struct StagePlane {
 //   dt come from the member (the state ping-pongs between M.q[cur] and
 //   M.q[1-cur]); the single-grid arguments are unused.
 //   MODE 2: tangent-linear stage (tl_dc2, tl_div used).
-//   ecols_a (single grid, not null): the stage also stores its output at
-//   the edge columns i = 1..3 and n-4..n-2 in the edge_cols layout, for the
-//   next gather -- p', v_xi, v_eta after stages 1-4 (the vector gather), all
-//   five fields after the last stage (the filter gather).
+//   ecols_a (single grid, not null; in an ensemble the member's ecols): the
+//   stage also stores its output at the edge columns i = 1..3 and n-4..n-2
+//   in the edge_cols layout, for the next gather -- p', v_xi, v_eta after
+//   stages 1-4 (the vector gather), all five fields after the last stage
+//   (the filter gather).
 //   PART: the tiles need not divide n.  The last tile of a row or column
 //   then has threads past the panel edge ("outside" threads): they store
 //   nothing, and those within three points of the edge bring the ghost values
@@ -430,8 +431,9 @@ __global__ __launch_bounds__(TI* TJ) SOLAR_STAGE_ATTR void k_stage(DevTables Tar
   const auto wo0 = RECON ? to_global(qwo_a) + (size_t)(P * 5) * nf : w0;
   // edge-column copy: layer l = i (W) or n-1-i (E); the others store none
   const int ecl = i <= 3 ? i : n - 1 - i;
-  const bool ec_on = !ENS && !rhs_mode && ecols_a != nullptr && ecl >= 1 && ecl <= 3 && own_in;
-  const auto ecw = to_global(ec_on ? ecols_a : qout_a);  // qout_a: never stored through when ec_on is false
+  S* const ecols = ENS ? reinterpret_cast<S*>(M[mem].ecols) : ecols_a;
+  const bool ec_on = !rhs_mode && ecols != nullptr && ecl >= 1 && ecl <= 3 && own_in;
+  const auto ecw = to_global(ecols);  // stored through only when ec_on
   const size_t ec_base = ec_on ? (size_t)(P * 2 + (i > 3)) * nr * edge_cols_plane(n) + (size_t)j * 3 + (ecl - 1) : 0;
 
   // Halo cells this thread fills: source pointer and plane stride, from the
@@ -1150,31 +1152,33 @@ __global__ __launch_bounds__(TI* TJ) __attribute__((amdgpu_waves_per_eu(6))) voi
   }
 }
 
-// Ensemble gather: the blocks of all members laid end to end (prefix[m] =
-// first block of member m, 24 * ceil(nr / gather_kc(nr)) blocks each);
-// blockIdx.y selects the field.
+// Ensemble gather over a group of members: their blocks laid end to end
+// (table[x] = first block of the group's member x, 24 *
+// gather_blocks_per_edge(nr, chunks) blocks each; table[nmem + x] = its index
+// in M); blockIdx.y selects the field.
 template <bool VEC, int NI>
-__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather_ens(const EnsMember* __restrict__ M, const int* __restrict__ prefix,
-                                                     int nmem, int cur) {
+__global__ __launch_bounds__(gather_threads_max(NI)) void k_gather_ens(const EnsMember* __restrict__ M, const int* __restrict__ table,
+                                                     int nmem, int cur, int cols, int chunks) {
   extern __shared__ double sline[];
   const int b = blockIdx.x;
   int lo = 0, hi = nmem - 1;  // last member whose first block is <= b
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= b)
+    if (table[mid] <= b)
       lo = mid;
     else
       hi = mid - 1;
   }
-  const EnsMember& E = M[lo];
-  const int li = b - prefix[lo], nr = E.T.nr, kc = gather_kc(nr);
+  const EnsMember& E = M[table[nmem + lo]];
+  const int li = b - table[lo], nr = E.T.nr, kc = gather_kc(nr, chunks);
   const int k0 = (li / 24) * kc, edge = li % 24, k1 = min(k0 + kc, nr);
   const long long fs = 12LL * nr * E.T.n * 3;
+  const double* ec = cols ? E.ecols : nullptr;
   if (VEC)
-    gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 4, E.gxp, E.gyp,
+    gather_edge<true, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], ec, 4, E.gxp, E.gyp,
                                   E.gxv, E.gyv, fs, sline, nullptr, 0.0);
   else
-    gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], nullptr, 0, E.fgx, E.fgy,
+    gather_edge<false, double, NI>(E.T, edge / 4, edge % 4, k0, k1, blockIdx.y, E.q[cur], ec, 0, E.fgx, E.fgy,
                                    nullptr, nullptr, fs, sline, nullptr, 0.0);
 }
 
@@ -1516,24 +1520,27 @@ hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, i
 }
 
 template <bool VEC, int NI>
-static hipError_t launch_gather_ens_v(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, int cur,
-                                      hipStream_t st) {
+static hipError_t launch_gather_ens_v(const EnsMember* M, const int* table, int nmem, int nblocks, int maxn, int cur,
+                                      int cols, int chunks, hipStream_t st) {
   const int nt = gather_threads(maxn, NI);
   const size_t lds = sizeof(double) * 2 * 3 * (VEC ? 3 : 1) * (size_t)maxn;
   hipError_t e = gather_lds(&k_gather_ens<VEC, NI>, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_gather_ens<VEC, NI>), dim3(nblocks, VEC ? 1 : 5), dim3(nt), lds, st, M, prefix, nmem, cur);
+  hipLaunchKernelGGL((k_gather_ens<VEC, NI>), dim3(nblocks, VEC ? 1 : 5), dim3(nt), lds, st, M, table, nmem, cur,
+                     cols, chunks);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
-                             hipStream_t st) {
+hipError_t launch_gather_ens(const EnsMember* M, const int* table, int nmem, int nblocks, int maxn, bool vec, int cur,
+                             bool cols, int chunks, hipStream_t st) {
   const bool two = 3 * maxn > 1024;  // n > 341: two strip points per thread
+  const int c = cols ? 1 : 0;
+  if (chunks < 1) return hipErrorInvalidValue;
   if (vec)
-    return two ? launch_gather_ens_v<true, 2>(M, prefix, nmem, nblocks, maxn, cur, st)
-               : launch_gather_ens_v<true, 1>(M, prefix, nmem, nblocks, maxn, cur, st);
-  return two ? launch_gather_ens_v<false, 2>(M, prefix, nmem, nblocks, maxn, cur, st)
-             : launch_gather_ens_v<false, 1>(M, prefix, nmem, nblocks, maxn, cur, st);
+    return two ? launch_gather_ens_v<true, 2>(M, table, nmem, nblocks, maxn, cur, c, chunks, st)
+               : launch_gather_ens_v<true, 1>(M, table, nmem, nblocks, maxn, cur, c, chunks, st);
+  return two ? launch_gather_ens_v<false, 2>(M, table, nmem, nblocks, maxn, cur, c, chunks, st)
+             : launch_gather_ens_v<false, 1>(M, table, nmem, nblocks, maxn, cur, c, chunks, st);
 }
 
 }  // namespace solar

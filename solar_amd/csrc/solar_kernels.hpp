@@ -30,13 +30,16 @@ struct DevTables {
 };
 
 // One member of a batched ensemble (FP64): its tables, state ping-pong
-// buffers, 2N accumulator, ghost strips and time step.
+// buffers, 2N accumulator, ghost strips, edge-column copies (written by
+// every stage, read by the next gather, edge_cols_elems(n, nr) values) and
+// time step.
 struct EnsMember {
   DevTables T;
   double* q[2];
   double* qw;
   double *gxp, *gyp, *gxv, *gyv;
   double *fgx, *fgy;
+  double* ecols;
   double dt;
 };
 
@@ -87,15 +90,21 @@ hipError_t launch_stage_ens(const EnsMember* M, const int* items, int nitems, in
                             int cur, hipStream_t st);
 hipError_t launch_filter_ens(const EnsMember* M, const int* items, int nitems, int grid, int ti, int tj, int cur,
                              hipStream_t st);
-// prefix[m]: first block of member m (24 * gather_blocks_per_edge(nr) blocks
-// each), nblocks in total; maxn: the largest n (block size and LDS).  vec:
-// the stage ghosts (p', velocity), else the filter's five scalars.
-hipError_t launch_gather_ens(const EnsMember* M, const int* prefix, int nmem, int nblocks, int maxn, bool vec, int cur,
-                             hipStream_t st);
+// A gather over a group of members: table[x] = first block of the group's
+// member x (24 * gather_blocks_per_edge(nr, chunks) blocks each), table[nmem +
+// x] = its index in M; nblocks in total; maxn: the largest n of the group
+// (block size and LDS).  vec:
+// the stage ghosts (p', velocity), else the filter's five scalars.  cols:
+// the W/E column strips come from the members' edge-column copies (every
+// gather but the first of a step, whose source no stage wrote).
+hipError_t launch_gather_ens(const EnsMember* M, const int* table, int nmem, int nblocks, int maxn, bool vec, int cur,
+                             bool cols, int chunks, hipStream_t st);
 // Planes per gather block (about 32 chunks per receiver edge), and gather
 // blocks per receiver edge.
-constexpr int gather_kc(int nr) { return (nr + 31) / 32; }
-constexpr int gather_blocks_per_edge(int nr) { return (nr + gather_kc(nr) - 1) / gather_kc(nr); }
+constexpr int gather_kc(int nr, int chunks = 32) { return (nr + chunks - 1) / chunks; }
+constexpr int gather_blocks_per_edge(int nr, int chunks = 32) {
+  return (nr + gather_kc(nr, chunks) - 1) / gather_kc(nr, chunks);
+}
 // Threads per gather block: one strip point each up to 1024 (n <= 341), else
 // two each (n <= 512).
 constexpr int gather_threads_max(int ni) { return ni == 1 ? 1024 : 768; }
